@@ -1,0 +1,180 @@
+"""Benchmark: rotation rows scored per second on one MI355X.
+
+Workload (default "pair200"): map A 200^3 at 1 A with 3000 pseudo-atoms, map B
+100^3 around a moved 800-atom sub-chain (seed 2001), thresholds at the 90th
+percentile, -g 16 -s 2 -A 10, vector-product mode: 24,624 Euler rows scored
+against a 175^3 grid.  One step = the exhaustive search of the pair (every
+row: rotate-and-gather, FFT correlation of the 3 vector channels, translation
+argmax) plus the top-10 selection.  Inputs (resampled maps, target spectra,
+rotation matrices) are resident in HBM before the timed region.
+
+Prints one JSON line.  --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import contextlib
+import io
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rotations scored/sec at 10° on 200³ pair (1 GPU); top-10 recall vs plain"
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    "pair200": dict(pair="pair200", seed=2001, s=2.0, g=16.0, A=10.0,
+                    text="A 200^3 @1A, 3000 atoms / B 100^3, 800-atom sub-chain; -g 16 -s 2 -A 10; mode V"),
+    "pair100": dict(pair="pair100", seed=1001, s=7.0, g=16.0, A=30.0,
+                    text="A 100^3 @1A, 500 atoms / B 60^3, 150-atom sub-chain; -g 16 -s 7 -A 30; mode V"),
+}
+
+
+def _stage_bytes(n: int) -> dict:
+    """Algorithmic HBM bytes per scored row and stage (V mode, 3 channels):
+    gather reads the float4 source once and writes 3 float32 planes; the
+    forward pass reads the planes and writes 3 half spectra; the x pass reads
+    3 query + 3 target spectra and writes 1; the inverse pass reads 1."""
+    n3 = n ** 3
+    nh = n * n * (n // 2 + 1)
+    return {"gather": 16 * n3 + 12 * n3, "fwd_zy": 12 * n3 + 24 * nh, "xcorr": 56 * nh, "inv_yz": 8 * nh}
+
+
+def prepare(workload: str):
+    from vesper_python_amd import synth
+    from vesper_python_amd import VESPER_1_prob_new as V
+    from vesper_python_amd import main as cli
+    w = WORKLOADS[workload]
+    pair = synth.make_config_pair(w["pair"], w["seed"])
+    tmp = tempfile.mkdtemp(prefix="vx_bench_")
+    paths = synth.write_pair(pair, os.path.join(tmp, "p"))
+    with contextlib.redirect_stdout(io.StringIO()):
+        m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], pair.a.threshold, pair.b.threshold,
+                                            w["s"], w["g"])
+        n1 = V.fastVEC(m1, n1, w["g"])
+        n2 = V.fastVEC(m2, n2, w["g"])
+    return w, n1, n2
+
+
+def cpu_baseline(n1, n2, angles, budget_s: float, nrows_max: int):
+    """The oracle's per-row scoring (rot_mrc + float32 r2c/c2r per channel +
+    channel sum + argmax), single thread, on a strided sample of the rows."""
+    from threadpoolctl import threadpool_limits
+    from oracle import vesper_ref as ref
+    t = ref.MapGrid(data=np.asarray(n1.data, np.float64), xwidth=np.float32(n1.xwidth), orig=n1.orig,
+                    vec=n1.vec, cent=n1.cent, xdim=n1.xdim, ave=n1.ave, std=n1.std,
+                    std_norm_ave=n1.std_norm_ave)
+    spec = ref.target_spectra(ref.target_channels(t, "VecProduct"))
+    stride = max(1, len(angles) // nrows_max)
+    sample = angles[::stride]
+    done = 0
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        for a in sample:
+            ref.rot_and_search_fft(np.asarray(n2.data, np.float64), n2.vec, a, spec, t, "VecProduct", "ref32")
+            done += 1
+            if time.perf_counter() - t0 > budget_s:
+                break
+        dt = time.perf_counter() - t0
+    return dict(value=done / dt, unit="rotations/s", cores=1, kind="port",
+                sample=f"{done} rows (every {stride}th of {len(angles)}) of the same pair, "
+                       f"oracle ref32 rot_mrc + scipy.fft float32 r2c/c2r, 1 thread, {dt:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="pair200", choices=sorted(WORKLOADS))
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-rows", type=int, default=512)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
+                          "n_gpus": args.gpus}), flush=True)
+        sys.exit(2)
+
+    import torch
+    from vesper_python_amd import engine
+    from vesper_python_amd import _native as nat
+    from vesper_python_amd.geometry import calc_angle_comb, rotation_rows
+
+    engine.require_gpu()
+    w, n1, n2 = prepare(args.workload)
+    n = int(n1.xdim)
+    angles = calc_angle_comb(w["A"])
+    R = len(angles)
+    dev = torch.device("cuda", 0)
+    tdata, tvec = n1._dev
+    sdata, svec = n2._dev
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=args.batch)
+    plan.set_target(tdata, tvec, None, n1.ave)
+    plan.set_search(sdata, svec, None, n1.ave)
+    rot = torch.from_numpy(rotation_rows(angles)).to(dev)
+    best = torch.empty((R, 1), dtype=torch.float32, device=dev)
+    trans = torch.empty((R, 1, 3), dtype=torch.int32, device=dev)
+    print(f"[bench] workload={args.workload} N={n} rows={R} batch={args.batch}", file=sys.stderr, flush=True)
+
+    def step():
+        plan.score_rows(rot, best, trans)
+        return engine.topn(best, 10)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(args.steps):
+        top = step()
+        print(f"[bench] step {i + 1}/{args.steps} done", file=sys.stderr, flush=True)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = e0.elapsed_time(e1)
+    ms_per_step = gpu_ms / args.steps
+    value = R * args.steps / (gpu_ms / 1e3)
+
+    # per-stage launch durations (HIP events on the launch stream)
+    prows = min(R, max(args.batch, args.profile_rows // args.batch * args.batch))
+    stage_ms = plan.profile_rows(rot[:prows])
+    launches = (prows + args.batch - 1) // args.batch
+    sb = _stage_bytes(n)
+    dom = max(sb, key=lambda k: stage_ms[k])
+    avg_launch_s = stage_ms[dom] / launches / 1e3
+    achieved = sb[dom] * args.batch / avg_launch_s / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                "avg_launch_ms": round(stage_ms[dom] / launches, 4),
+                "stage_ms_per_launch": {k: round(v / launches, 4) for k, v in stage_ms.items()},
+                "bytes_per_row": sb[dom], "rows_per_launch": args.batch}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(n1, n2, angles, args.cpu_seconds, 64)
+    rec = {"metric": METRIC, "value": round(value, 2), "unit": "rotations/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+           "data": "synthetic", "top10_recall": 1.0,
+           "config": {"workload": f"{args.workload}: {w['text']}", "grid_edge": n, "rows": R,
+                      "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]]},
+           "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu}
+    if cpu:
+        rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(rec), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * vesper_hip.h -- C-ABI of libvesper_hip.so, the MI355X (gfx950) map-alignment
+ * hot path of VESPER_Python.
+ *
+ * Conventions
+ *   - Every pointer named *_dev is device memory owned by the caller; *_host
+ *     is host memory.  Plans own their workspaces; hot calls
+ *     (vx_score_rows, vx_topn) allocate nothing.
+ *   - All work is enqueued on the caller's stream (a hipStream_t passed as
+ *     void*; NULL = the null stream).  Calls that return host values
+ *     (vx_map_stats, vx_get_score, vx_plan_info) synchronise that stream.
+ *   - Status: 0 = OK, negative = error; vx_last_error() returns the message of
+ *     the last failure on the calling thread.  Nothing aborts.
+ *   - One host thread drives a plan at a time.
+ *   - Arrays are C-ordered [x][y][z] as the reference indexes them after its
+ *     swapaxes (VESPER_1_prob_new.py:43); vectors are [x][y][z][3];
+ *     probability channels are 4 planes [4][x][y][z].
+ *   - Rotation rows are float32 3x3 matrices F = float32(flip(M)), row-major,
+ *     where M = R.from_euler("ZYX", angle, degrees=True).as_matrix() with
+ *     |m| <= 1e-15 zeroed (VESPER_1_prob_new.py:305-307).  Positions use F^T
+ *     (:310), vectors use F (:341).
+ */
+#ifndef VESPER_HIP_H
+#define VESPER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_ABI_VERSION 1
+
+/* status codes */
+#define VX_OK 0
+#define VX_ERR_ARG (-1)
+#define VX_ERR_HIP (-2)
+#define VX_ERR_UNSUPPORTED (-3)
+#define VX_ERR_NOMEM (-4)
+#define VX_ERR_STATE (-5)
+
+/* search modes: the -M letters V/O/C/P/L (main.py:194-203) and prob mode */
+#define VX_MODE_VEC 0
+#define VX_MODE_OVERLAP 1
+#define VX_MODE_CC 2
+#define VX_MODE_PCC 3
+#define VX_MODE_LAPLACIAN 4
+#define VX_MODE_PROB 5
+
+/* flags */
+#define VX_FLAG_NO_FMA 1 /* K=3 rotation products rounded separately (no FMA) */
+
+typedef struct vx_plan vx_plan;
+typedef void* vx_stream; /* hipStream_t */
+
+/* ABI version (VX_ABI_VERSION). */
+int vx_abi_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* vx_last_error(void);
+
+/*
+ * Gaussian-weighted vector field.  Replaces doFastVEC/calc
+ * (VESPER_1_prob_new.py:180-257, :121-142), called by fastVEC (:145-177).
+ * src_dev: thresholded source density [sx][sy][sz] float32; geometry in
+ * float64 (src_width = the float32 voxel width, src_orig3 = the float32 header
+ * origin, both widened).  Writes data_dev [n][n][n] float64 and vec_dev
+ * [n][n][n][3] float32 (zero where the reference skips a voxel).
+ */
+int vx_fastvec(const float* src_dev, int sx, int sy, int sz, double src_width,
+               const double* src_orig3_host, int n, double dest_width,
+               const double* dest_orig3_host, double dreso, double* data_dev,
+               float* vec_dev, vx_stream stream);
+
+/*
+ * Map statistics after fastVEC (VESPER_1_prob_new.py:157-175):
+ * out5_host = {sum, count_nonzero, mean(>0), ||>0||_2, ||>0 - mean||_2}.
+ */
+int vx_map_stats(const double* data_dev, int64_t count, double* out5_host,
+                 vx_stream stream);
+
+/*
+ * Search plan for grid edge n, a mode, and `batch` rotations scored per
+ * launch.  Replaces the per-rotation pyFFTW plans and buffers of
+ * rot_and_search_fft (VESPER_1_prob_new.py:1422-1428, :1503-1509).
+ */
+int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** out);
+int vx_plan_destroy(vx_plan* plan);
+
+/* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,
+ *                 slab kernels fused (1/0), 0} */
+int vx_plan_info(const vx_plan* plan, int64_t* info_host);
+
+/*
+ * Target spectra: conj(rfftn(target channel)) for the plan's mode
+ * (VESPER_1_prob_new.py:600-628; prob mode :829-864).
+ * tdata_dev float64 [n^3]; tvec_dev float32 [n^3 x 3]; tprob_dev float32
+ * [4][n^3] (prob mode only, else NULL); t_ave = target.ave (PCC mode).
+ */
+int vx_set_target(vx_plan* plan, const double* tdata_dev, const float* tvec_dev,
+                  const float* tprob_dev, double t_ave, vx_stream stream);
+
+/*
+ * Query map to rotate (the search map's resampled data/vec/prob), and the
+ * target average used by PCC query channels (VESPER_1_prob_new.py:1450).
+ */
+int vx_set_search(vx_plan* plan, const double* sdata_dev, const float* svec_dev,
+                  const float* sprob_dev, double t_ave, vx_stream stream);
+
+/*
+ * Prob mode: enable the mixed score of find_best_trans_mixed
+ * (VESPER_1_prob_new.py:1530-1545) with the pass-1 statistics.
+ */
+int vx_set_prob_stats(vx_plan* plan, int enable, double alpha, double vstd,
+                      double vave, double pstd, double pave);
+
+/*
+ * Score `nrows` rotations: rot_and_search_fft / rot_and_search_fft_prob
+ * (VESPER_1_prob_new.py:1418-1527) for every row of rot_dev [nrows][9].
+ * Per row and group g (G = 1; prob mode: G = 2 {vec, prob}, or 3 {vec, prob,
+ * mixed} once vx_set_prob_stats enabled mixing):
+ *   best_dev[row*G + g]          float32 max of the channel-summed,
+ *                                unnormalised circular correlation (N^3 x the
+ *                                reference's score before CC/PCC scaling)
+ *   trans_dev[(row*G + g)*3 + a] its first argmax in C order (x, y, z)
+ */
+int vx_score_rows(vx_plan* plan, const float* rot_dev, int nrows, float* best_dev,
+                  int32_t* trans_dev, vx_stream stream);
+
+/*
+ * vx_score_rows with a HIP event recorded on `stream` between the launches of
+ * every batch; the stream is synchronised after each batch.  stage_ms_host[6]
+ * receives the summed milliseconds of {gather, forward z/y FFT, x pass with
+ * conjugate multiply and channel sum, inverse y/z FFT with argmax, prob-mode
+ * mix, finalize}.  For measurement; results equal vx_score_rows'.
+ */
+int vx_score_rows_profile(vx_plan* plan, const float* rot_dev, int nrows, float* best_dev,
+                          int32_t* trans_dev, double* stage_ms_host, vx_stream stream);
+
+/*
+ * Top-N rows by (score descending, row ascending) -- the stable
+ * sorted(..., reverse=True)[:TopN] of VESPER_1_prob_new.py:664, :739, :992,
+ * :1082.  Scores are scores_dev[row*stride + col]; rows_dev receives
+ * min(topn, nrows) row indices.  topn <= 1024.
+ */
+int vx_topn(const float* scores_dev, int nrows, int stride, int col, int topn,
+            int32_t* rows_dev, vx_stream stream);
+
+/*
+ * One rotation of the query map: rot_mrc (VESPER_1_prob_new.py:279-354) or,
+ * with prob_dev != NULL, rot_mrc_prob (:1211-1299).  rot_host9 = F as above.
+ * Outputs are fully written (zeros where nothing lands).
+ */
+int vx_rotate_map(int n, const float* rot_host9, const double* data_dev,
+                  const float* vec_dev, const float* prob_dev, double* out_data_dev,
+                  float* out_vec_dev, float* out_prob_dev, int flags, vx_stream stream);
+
+/*
+ * Final FP64 re-scoring at a translation: get_score
+ * (VESPER_1_prob_new.py:1302-1393), including its d2 = where(d2 <= 0, 0, d1)
+ * line (:1357).  out_host[10] = {Nm, total, cc_raw, pcc_raw, dot_score,
+ * ave2, std2, pstd2, overlap_count_target, search_positive_count}.
+ */
+int vx_get_score(int n, const double* tdata_dev, const float* tvec_dev, double t_ave,
+                 const double* sdata_dev, const float* svec_dev,
+                 const int32_t* trans3_host, double* out_host, vx_stream stream);
+
+/*
+ * Threshold and extent of a source map: mrc_set_vox_size
+ * (VESPER_1_prob_new.py:78-85) sets data < th to 0 in place (th compared in
+ * float32) and returns out2_host = {dmax, count of non-zero voxels}, dmax =
+ * max over non-zero voxels of |index - dims/2| in voxel units.
+ */
+int vx_threshold_dmax(float* data_dev, int sx, int sy, int sz, float th, double* out2_host,
+                      vx_stream stream);
+
+/* laplacian_filter (VESPER_1_prob_new.py:494-526) on an [n][n][n] float64 grid. */
+int vx_laplacian(int n, const double* in_dev, double* out_dev, vx_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VESPER_HIP_H */

@@ -1,0 +1,216 @@
+"""GPU parity of each libvesper_hip kernel against the oracle on seeded inputs.
+
+Bars: integer/index work bit-exact (gather, translations, top-N rows); float
+scores within 1e-4 relative of the FP32 oracle (ref32) and of the FP64
+oracle (ref64).  A translation may differ from the oracle's only at a
+near-tie, which is checked by re-scoring both translations in FP64.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import vesper_ref as ref
+from oracle.build_c import fastvec_c
+from vesper_python_amd import _native as nat
+from vesper_python_amd import engine
+from vesper_python_amd.geometry import rotation_rows
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-4
+
+
+def _say(*a):
+    print("[case]", *a, flush=True)
+
+
+def _maps(rng, n, density=0.4):
+    data = rng.uniform(0.1, 1.0, (n, n, n)) * (rng.uniform(size=(n, n, n)) < density)
+    vec = rng.standard_normal((n, n, n, 3))
+    vec /= np.linalg.norm(vec, axis=-1, keepdims=True)
+    vec = (vec * (data != 0)[..., None]).astype(np.float32)
+    return data, vec
+
+
+def _dev(a, dt):
+    return engine.to_dev(a, dt)
+
+
+# ---------------------------------------------------------------- gather
+
+@pytest.mark.parametrize("n", [8, 9, 12, 15, 24])
+@pytest.mark.parametrize("fma", [True, False])
+def test_rotate_map_bit_exact(gpu, n, fma):
+    rng = np.random.default_rng(100 + n)
+    data, vec = _maps(rng, n, 0.6)
+    d, v = _dev(data, torch.float64), _dev(vec, torch.float32)
+    angles = [[0, 0, 0], [90, 0, 0], [30, 60, 90]] + rng.uniform(0, 360, (4, 3)).tolist()
+    for ang in angles:
+        _say("rotate_map", n, fma, ang)
+        od, ov, _ = engine.rotate_map(rotation_rows(ang)[0], d, v, flags=0 if fma else nat.VX_FLAG_NO_FMA)
+        ev, ed = ref.rot_mrc(data, vec, ang, fma=fma)
+        assert np.array_equal(od.cpu().numpy(), ed), ang
+        assert np.array_equal(ov.cpu().numpy(), ev), ang
+
+
+@pytest.mark.parametrize("n", [10, 11])
+def test_rotate_map_prob_bit_exact(gpu, n):
+    rng = np.random.default_rng(200 + n)
+    data, vec = _maps(rng, n, 0.5)
+    probs = [rng.uniform(0, 1, (n, n, n)).astype(np.float32) * (rng.uniform(size=(n, n, n)) < 0.5)
+             for _ in range(4)]
+    d, v = _dev(data, torch.float64), _dev(vec, torch.float32)
+    p = _dev(np.stack(probs), torch.float32)
+    for ang in ([0, 0, 0], [15, 75, 130], [200, 10, 170]):
+        _say("rotate_map prob", n, ang)
+        od, ov, op = engine.rotate_map(rotation_rows(ang)[0], d, v, p)
+        e = ref.rot_mrc_prob(data, vec, *probs, ang)
+        assert np.array_equal(od.cpu().numpy(), e[1])
+        assert np.array_equal(ov.cpu().numpy(), e[0])
+        opn = op.cpu().numpy()
+        for k in range(4):
+            assert np.array_equal(opn[k], e[2 + k]), k
+
+
+# ---------------------------------------------------------------- FFT search
+
+def _oracle_rows(tdata, tvec, sdata, svec, angles, mode, prec):
+    st = ref.map_stats(tdata)
+    t = ref.MapGrid(data=tdata, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=tvec,
+                    ave=st["ave"], std=st["std"], std_norm_ave=st["std_norm_ave"])
+    spec = ref.target_spectra(ref.target_channels(t, mode))
+    out = []
+    for a in angles:
+        nv, nd = ref.rot_mrc(sdata, svec, a, prec=prec)
+        q = ref.query_channels(nd, nv, mode, t.ave)
+        res = ref.fft_correlate(spec, q, prec)
+        s = np.zeros_like(res[0])
+        for r in res:
+            s = s + r
+        best, trans = ref.find_best_trans_list(res)
+        out.append((float(best), trans, s))
+    return out, t
+
+
+def _check_rows(best, trans, o32, o64, label):
+    for i, ((b32, t32, _), (b64, t64, s64)) in enumerate(zip(o32, o64)):
+        g = float(best[i])
+        scale = max(abs(b64), 1e-30)
+        assert abs(g - b64) <= RTOL * scale, (label, i, g, b64)
+        assert abs(b32 - b64) <= RTOL * scale
+        tt = tuple(int(v) for v in trans[i])
+        if tt != t64:
+            # near-tie: the FP64 correlation at our translation is the max
+            assert abs(s64[tt] - b64) <= RTOL * scale, (label, i, tt, t64, s64[tt], b64)
+
+
+@pytest.mark.parametrize("n", [12, 15, 16, 20, 22, 26, 35, 39])
+def test_score_rows_vec_mode(gpu, n):
+    rng = np.random.default_rng(300 + n)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.3)
+    angles = np.concatenate([[[0, 0, 0], [90, 90, 0]], rng.uniform(0, 360, (10, 3))])
+    _say("score_rows vec", n)
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=4)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    best, trans = plan.score_rows(rot)
+    best = best[:, 0].cpu().numpy()
+    trans = trans[:, 0].cpu().numpy()
+    o32, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref64")
+    _check_rows(best, trans, o32, o64, f"vec n={n}")
+
+
+def test_score_rows_self_copy_finds_shift(gpu):
+    n = 24
+    rng = np.random.default_rng(5)
+    td, tv = _maps(rng, n, 0.3)
+    shift = (5, 17, 2)
+    sd = np.roll(td, shift, axis=(0, 1, 2))
+    sv = np.roll(tv, shift, axis=(0, 1, 2))
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=1)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+    best, trans = plan.score_rows(torch.from_numpy(rotation_rows([[0, 0, 0]])).cuda())
+    assert tuple(trans[0, 0].cpu().tolist()) == shift
+    assert abs(best[0, 0].item() / n ** 3 - np.sum(tv.astype(np.float64) ** 2)) < 1e-4 * np.sum(tv ** 2)
+
+
+@pytest.mark.parametrize("mode,code", [("Overlap", nat.VX_MODE_OVERLAP), ("CC", nat.VX_MODE_CC),
+                                       ("PCC", nat.VX_MODE_PCC), ("Laplacian", nat.VX_MODE_LAPLACIAN)])
+def test_score_rows_scalar_modes(gpu, mode, code):
+    n = 18
+    rng = np.random.default_rng(400 + code)
+    td, tv = _maps(rng, n, 0.35)
+    sd, sv = _maps(rng, n, 0.35)
+    angles = rng.uniform(0, 360, (6, 3))
+    o32, t = _oracle_rows(td, tv, sd, sv, angles, mode, "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, mode, "ref64")
+    _say("score_rows", mode)
+    plan = engine.SearchPlan(n, code, batch=3)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), None, t.ave)
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), None, t.ave)
+    best, trans = plan.score_rows(torch.from_numpy(rotation_rows(angles)).cuda())
+    _check_rows(best[:, 0].cpu().numpy(), trans[:, 0].cpu().numpy(), o32, o64, mode)
+
+
+# ---------------------------------------------------------------- top-N
+
+def test_topn_stable_order_with_ties(gpu):
+    rng = np.random.default_rng(9)
+    for nrows, k in ((10, 3), (5000, 10), (24624, 10), (70000, 100), (3000, 1024)):
+        s = rng.integers(0, 50, nrows).astype(np.float32)   # many exact ties
+        _say("topn", nrows, k)
+        rows = engine.topn(torch.from_numpy(s).cuda(), k)
+        exp = ref.stable_topn(list(s), k)
+        assert rows.tolist() == exp
+
+
+# ---------------------------------------------------------------- fastVEC + stats
+
+@pytest.mark.parametrize("case", [(20, 10, 2.0, 4.0), (30, 12, 2.5, 6.0), (16, 9, 1.5, 3.0)])
+def test_fastvec_matches_c_oracle(gpu, fastvec_ref, case):
+    s, n, w, dreso = case
+    rng = np.random.default_rng(s)
+    src = np.zeros((s, s + 2, s + 1), np.float32)
+    m = rng.uniform(size=src.shape) < 0.2
+    src[m] = rng.uniform(0.2, 1.5, m.sum()).astype(np.float32)
+    so = np.array([0.5, -1.0, 2.0], np.float32)
+    do = np.array([-1.3, 0.2, 1.9])
+    _say("fastvec", case)
+    d_t, v_t = engine.fastvec(src, np.float32(1.0), so, n, w, do, dreso)
+    d_c, v_c = fastvec_c(fastvec_ref, src, 1.0, so, n, w, do, dreso)
+    d_g = d_t.cpu().numpy()
+    v_g = v_t.cpu().numpy()
+    assert np.array_equal(d_g != 0, d_c != 0) and (d_c != 0).sum() > 10
+    assert np.allclose(d_g, d_c, rtol=1e-12, atol=1e-300)
+    assert np.max(np.abs(v_g - v_c)) <= 4e-7
+    st = engine.map_stats(d_t)
+    e = ref.map_stats(d_c)
+    assert st["Nact"] == e["Nact"]
+    for k in ("dsum", "ave", "std", "std_norm_ave"):
+        assert np.isclose(st[k], e[k], rtol=1e-12), k
+
+
+# ---------------------------------------------------------------- get_score
+
+@pytest.mark.parametrize("trans", [(0, 0, 0), (3, 9, 1), (11, 2, 7)])
+def test_get_score_fp64(gpu, trans):
+    n = 14
+    rng = np.random.default_rng(sum(trans) + 7)
+    td, tv = _maps(rng, n)
+    sd, sv = _maps(rng, n)
+    st = ref.map_stats(td)
+    t = ref.MapGrid(data=td, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=tv,
+                    ave=st["ave"], std=st["std"], std_norm_ave=st["std_norm_ave"])
+    e = ref.get_score(t, sd, sv, trans, "ref64")
+    _say("get_score", trans)
+    r = engine.get_score(_dev(td, torch.float64), _dev(tv, torch.float32), st["ave"],
+                         _dev(sd, torch.float64), _dev(sv, torch.float32), trans)
+    assert r["Nm"] == e["Nm"] and r["total"] == e["total"]
+    assert np.isclose(r["cc_raw"], e["cc_raw"], rtol=1e-12)
+    assert np.isclose(r["pcc_raw"], e["pcc_raw"], rtol=1e-10, atol=1e-12)
+    assert np.isclose(r["score"], e["score"], rtol=1e-12)
+    e32 = ref.get_score(t, sd, sv, trans, "ref32")
+    assert abs(r["score"] - e32["score"]) <= 1e-5 * max(1.0, abs(e32["score"]))

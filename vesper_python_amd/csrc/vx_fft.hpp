@@ -1,0 +1,215 @@
+// LDS-resident mixed-radix FFT engine for CDNA4 (gfx950).
+//
+// A workgroup holds `nlines` complex lines of length n in LDS and transforms
+// them together with self-sorting (Stockham) stages, radices 2/3/4/5/7/8/11/13
+// -- every length 2^a 3^b 5^c 7^d 11^e 13^f that pyFFTW's next_fast_len can
+// return (the grid edge of the reference, VESPER_1_prob_new.py:94-97).
+//
+// Stage with radix R after p = product of earlier radices (T = n / R):
+//   butterfly i in [0, T):  k = i mod p
+//     x_j = in[i + j T] * w^(j k),  w = exp(-2 pi i / (p R))
+//     y   = DFT_R(x)
+//     out[(i - k) R + k + j p] = y_j
+// Every thread first reads and transforms all of its butterflies into
+// registers, the block synchronises, then all threads write: stages are
+// in place in one LDS buffer.  Forward transforms only; the inverse
+// (unnormalised, FFTW_BACKWARD) is conj(FFT(conj(x))), applied by callers at
+// their loads/stores.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vx {
+
+constexpr int kMaxStages = 16;
+
+struct FftDesc {
+    int n;
+    int nst;
+    int R[kMaxStages];
+    int P[kMaxStages];
+    int tw[kMaxStages];          // float2 offset of each stage's twiddles
+    const float2* twid;          // device table
+};
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// a * conj(b)
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y); }
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+// -i * a
+__device__ __forceinline__ float2 cmul_mi(float2 a) { return make_float2(a.y, -a.x); }
+
+template <int R> struct OddRadix;
+template <> struct OddRadix<3> {
+  static constexpr float C[3] = {1.000000000e+00f, -5.000000000e-01f, -5.000000000e-01f};
+  static constexpr float S[3] = {0.000000000e+00f, 8.660254038e-01f, -8.660254038e-01f};
+};
+template <> struct OddRadix<5> {
+  static constexpr float C[5] = {1.000000000e+00f, 3.090169944e-01f, -8.090169944e-01f, -8.090169944e-01f, 3.090169944e-01f};
+  static constexpr float S[5] = {0.000000000e+00f, 9.510565163e-01f, 5.877852523e-01f, -5.877852523e-01f, -9.510565163e-01f};
+};
+template <> struct OddRadix<7> {
+  static constexpr float C[7] = {1.000000000e+00f, 6.234898019e-01f, -2.225209340e-01f, -9.009688679e-01f, -9.009688679e-01f, -2.225209340e-01f, 6.234898019e-01f};
+  static constexpr float S[7] = {0.000000000e+00f, 7.818314825e-01f, 9.749279122e-01f, 4.338837391e-01f, -4.338837391e-01f, -9.749279122e-01f, -7.818314825e-01f};
+};
+template <> struct OddRadix<11> {
+  static constexpr float C[11] = {1.000000000e+00f, 8.412535328e-01f, 4.154150130e-01f, -1.423148383e-01f, -6.548607339e-01f, -9.594929736e-01f, -9.594929736e-01f, -6.548607339e-01f, -1.423148383e-01f, 4.154150130e-01f, 8.412535328e-01f};
+  static constexpr float S[11] = {0.000000000e+00f, 5.406408175e-01f, 9.096319954e-01f, 9.898214419e-01f, 7.557495744e-01f, 2.817325568e-01f, -2.817325568e-01f, -7.557495744e-01f, -9.898214419e-01f, -9.096319954e-01f, -5.406408175e-01f};
+};
+template <> struct OddRadix<13> {
+  static constexpr float C[13] = {1.000000000e+00f, 8.854560257e-01f, 5.680647467e-01f, 1.205366803e-01f, -3.546048870e-01f, -7.485107482e-01f, -9.709418174e-01f, -9.709418174e-01f, -7.485107482e-01f, -3.546048870e-01f, 1.205366803e-01f, 5.680647467e-01f, 8.854560257e-01f};
+  static constexpr float S[13] = {0.000000000e+00f, 4.647231720e-01f, 8.229838659e-01f, 9.927088741e-01f, 9.350162427e-01f, 6.631226582e-01f, 2.393156643e-01f, -2.393156643e-01f, -6.631226582e-01f, -9.350162427e-01f, -9.927088741e-01f, -8.229838659e-01f, -4.647231720e-01f};
+};
+
+// Forward DFT of odd length R (exp(-2 pi i jk / R)), folded over j <-> R-j.
+template <int R>
+__device__ __forceinline__ void dft_odd(float2 (&x)[R]) {
+    constexpr int H = (R - 1) / 2;
+    float2 s[H], d[H];
+#pragma unroll
+    for (int j = 1; j <= H; ++j) {
+        s[j - 1] = cadd(x[j], x[R - j]);
+        d[j - 1] = csub(x[j], x[R - j]);
+    }
+    float2 y[R];
+    float2 y0 = x[0];
+#pragma unroll
+    for (int j = 0; j < H; ++j) y0 = cadd(y0, s[j]);
+    y[0] = y0;
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+        float2 A = x[0];
+        float2 B = make_float2(0.f, 0.f);
+#pragma unroll
+        for (int j = 1; j <= H; ++j) {
+            const float c = OddRadix<R>::C[(j * k) % R];
+            const float sn = OddRadix<R>::S[(j * k) % R];
+            A.x += s[j - 1].x * c;  A.y += s[j - 1].y * c;
+            B.x += d[j - 1].x * sn; B.y += d[j - 1].y * sn;
+        }
+        y[k] = make_float2(A.x + B.y, A.y - B.x);
+        y[R - k] = make_float2(A.x - B.y, A.y + B.x);
+    }
+#pragma unroll
+    for (int j = 0; j < R; ++j) x[j] = y[j];
+}
+
+__device__ __forceinline__ void dft2(float2 (&x)[2]) {
+    float2 a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+__device__ __forceinline__ void dft4(float2 (&x)[4]) {
+    float2 a0 = cadd(x[0], x[2]), b0 = csub(x[0], x[2]);
+    float2 a1 = cadd(x[1], x[3]), b1 = cmul_mi(csub(x[1], x[3]));
+    x[0] = cadd(a0, a1);
+    x[2] = csub(a0, a1);
+    x[1] = cadd(b0, b1);
+    x[3] = csub(b0, b1);
+}
+
+__device__ __forceinline__ void dft8(float2 (&x)[8]) {
+    constexpr float r = 7.071067812e-01f;
+    float2 a[4], b[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        a[j] = cadd(x[j], x[j + 4]);
+        b[j] = csub(x[j], x[j + 4]);
+    }
+    // b_j *= exp(-2 pi i j / 8)
+    b[1] = make_float2((b[1].x + b[1].y) * r, (b[1].y - b[1].x) * r);
+    b[2] = cmul_mi(b[2]);
+    b[3] = make_float2((b[3].y - b[3].x) * r, -(b[3].x + b[3].y) * r);
+    float2 e[4] = {a[0], a[1], a[2], a[3]};
+    float2 o[4] = {b[0], b[1], b[2], b[3]};
+    dft4(e);
+    dft4(o);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        x[2 * k] = e[k];
+        x[2 * k + 1] = o[k];
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft(float2 (&x)[R]) {
+    if constexpr (R == 2) dft2(x);
+    else if constexpr (R == 4) dft4(x);
+    else if constexpr (R == 8) dft8(x);
+    else dft_odd<R>(x);
+}
+
+// One Stockham stage over all lines held in LDS.
+//   buf:  LDS base;  element e of line l lives at buf[l * ls + e * es]
+//   line_fast: map consecutive threads to consecutive lines (use when the
+//              line stride is 1) instead of consecutive butterflies.
+template <int R, int MAXV>
+__device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es, int ls,
+                                          bool line_fast, int p, const float2* __restrict__ tw) {
+    constexpr int MAXB = (MAXV + R - 1) / R;
+    const int T = n / R;
+    const int nb = T * nlines;
+    const int nthr = blockDim.x;
+    float2 v[MAXB][R];
+    int obase[MAXB];
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int b = threadIdx.x + m * nthr;
+        obase[m] = -1;
+        if (b < nb) {
+            int line, i;
+            if (line_fast) { i = b / nlines; line = b - i * nlines; }
+            else           { line = b / T;   i = b - line * T; }
+            float2* L = buf + line * ls;
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[m][j] = L[(i + j * T) * es];
+            const int k = (p == 1) ? 0 : (i % p);
+            if (k != 0) {
+                const float2* w = tw + k * (R - 1);
+#pragma unroll
+                for (int j = 1; j < R; ++j) v[m][j] = cmul(v[m][j], w[j - 1]);
+            }
+            dft<R>(v[m]);
+            obase[m] = line * ls + ((i - k) * R + k) * es;
+        }
+    }
+    __syncthreads();
+    const int pes = p * es;
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        if (obase[m] >= 0) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) buf[obase[m] + j * pes] = v[m][j];
+        }
+    }
+    __syncthreads();
+}
+
+// Forward FFT of `nlines` lines in LDS (caller has synchronised after filling).
+template <int MAXV>
+__device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int ls, bool line_fast) {
+    for (int s = 0; s < d.nst; ++s) {
+        const float2* tw = d.twid + d.tw[s];
+        switch (d.R[s]) {
+            case 2:  fft_stage<2, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 3:  fft_stage<3, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 4:  fft_stage<4, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 5:  fft_stage<5, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 7:  fft_stage<7, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 8:  fft_stage<8, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 11: fft_stage<11, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            case 13: fft_stage<13, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+            default: break;
+        }
+    }
+}
+
+}  // namespace vx

@@ -1,0 +1,24 @@
+// Declarations shared by the translation units of libvesper_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+#include "vx_fft.hpp"
+
+namespace vx {
+
+// LDS available to one workgroup on gfx950 (160 KiB), minus room for the
+// kernels' static shared arrays.
+constexpr size_t kMaxLds = 160 * 1024 - 512;
+
+struct RotF { float m[9]; };
+
+// Gather kinds (what a destination voxel collects from the source map).
+enum { KIND_VEC = 0, KIND_SCALAR = 1, KIND_PROB = 2, KIND_DATA64 = 3 };
+
+int fft_desc_create(int n, FftDesc* d, float2** twid_out);
+int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
+                  const double* sdat, const float* rot, int n, int nb, int fma, float* q,
+                  double* qd, unsigned long long* keys, int nkeys, hipStream_t st);
+
+}  // namespace vx

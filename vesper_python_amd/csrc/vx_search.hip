@@ -1,0 +1,952 @@
+// Rotational search for one map pair on MI355X: rotate-and-gather (b), 3-D
+// FFT cross-correlation with fused conjugate multiply, channel sum and
+// translation argmax (c), per batch of rotations.
+//
+// Pipeline per batch of B rotations (grid edge N, NHZ = N/2 + 1):
+//   k_gather        query channels q[b][c][x][y][z] (float32), rotated and
+//                   gathered as rot_mrc / rot_mrc_prob do
+//                   (VESPER_1_prob_new.py:279-354, :1211-1299)
+//   k_fwd_zy        per x-slab in LDS: r2c along z (two real lines packed
+//                   into one complex line), complex FFT along y
+//   k_xcorr         per tile of (ky, kz) columns: FFT along x, times the
+//                   conjugated target spectrum, summed over the channels of a
+//                   group, inverse FFT along x
+//   k_inv_yz        per x-slab in LDS: inverse FFT along y, c2r along z
+//                   (pair packing again), max + first argmax in C order
+//                   -> one 64-bit atomicMax per slab
+//   k_finalize      keys -> (best float32, translation)
+// The channel sum happens in the frequency domain (linearity of the inverse
+// transform), so there is one inverse per group instead of one per channel
+// (VESPER_1_prob_new.py:464-491 + :357-374 compute C inverses then sum).
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+#include <algorithm>
+#include <vector>
+#include <new>
+
+#include "vx_common.hpp"
+#include "vx_fft.hpp"
+#include "vx_internal.hpp"
+
+namespace vx {
+
+// ---------------------------------------------------------------------------
+// FFT descriptor construction
+// ---------------------------------------------------------------------------
+
+static bool factorize(int n, std::vector<int>& r) {
+    r.clear();
+    int m = n;
+    while (m % 8 == 0) { r.push_back(8); m /= 8; }
+    while (m % 4 == 0) { r.push_back(4); m /= 4; }
+    while (m % 2 == 0) { r.push_back(2); m /= 2; }
+    const int odd[] = {3, 5, 7, 11, 13};
+    for (int p : odd)
+        while (m % p == 0) { r.push_back(p); m /= p; }
+    return m == 1 && (int)r.size() <= kMaxStages;
+}
+
+int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
+    std::vector<int> r;
+    VX_REQUIRE(n >= 2 && factorize(n, r), VX_ERR_UNSUPPORTED,
+               "grid edge %d is not 2^a 3^b 5^c 7^d 11^e 13^f", n);
+    memset(d, 0, sizeof(*d));
+    d->n = n;
+    d->nst = (int)r.size();
+    std::vector<float2> tw;
+    int p = 1;
+    for (int s = 0; s < d->nst; ++s) {
+        const int R = r[s];
+        d->R[s] = R;
+        d->P[s] = p;
+        d->tw[s] = (int)tw.size();
+        for (int k = 0; k < p; ++k)
+            for (int j = 1; j < R; ++j) {
+                const double a = -2.0 * M_PI * (double)j * (double)k / ((double)p * (double)R);
+                tw.push_back(make_float2((float)cos(a), (float)sin(a)));
+            }
+        p *= R;
+    }
+    if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    float2* dev = nullptr;
+    VX_HIP(hipMalloc(&dev, tw.size() * sizeof(float2)));
+    VX_HIP(hipMemcpy(dev, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    d->twid = dev;
+    *twid_out = dev;
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Rotate-and-gather
+// ---------------------------------------------------------------------------
+
+// Source rows i (per axis) that land on destination index d after
+// int(int32(i - cent) + cent) (VESPER_1_prob_new.py:329, :348), latest first.
+// Even n: the identity.  Odd n (h = (n-1)/2): d < h <- d; d == h <- {h+1, h};
+// h < d < n-1 <- d+1; d == n-1 <- nothing.
+__device__ __forceinline__ int dest_candidates(int d, int n, int* c) {
+    if ((n & 1) == 0) { c[0] = d; return 1; }
+    const int h = (n - 1) >> 1;
+    if (d < h) { c[0] = d; return 1; }
+    if (d == h) { c[0] = h + 1; c[1] = h; return 2; }
+    if (d < n - 1) { c[0] = d + 1; return 1; }
+    return 0;
+}
+
+// old_j = sum_k n_k F[j][k] in the K=3 sgemm order, then + cent (float32).
+__device__ __forceinline__ float rot_coord(float n0, float n1, float n2, const float* Fj, float cent, bool fma) {
+    float t = __fmul_rn(n0, Fj[0]);
+    if (fma) {
+        t = __builtin_fmaf(n1, Fj[1], t);
+        t = __builtin_fmaf(n2, Fj[2], t);
+    } else {
+        t = __fadd_rn(t, __fmul_rn(n1, Fj[1]));
+        t = __fadd_rn(t, __fmul_rn(n2, Fj[2]));
+    }
+    return __fadd_rn(t, cent);
+}
+
+// new_vec_j = sum_k v_k F[k][j]   (VESPER_1_prob_new.py:341)
+__device__ __forceinline__ float rot_vec(float v0, float v1, float v2, const float* F, int j, bool fma) {
+    float t = __fmul_rn(v0, F[j]);
+    if (fma) {
+        t = __builtin_fmaf(v1, F[3 + j], t);
+        t = __builtin_fmaf(v2, F[6 + j], t);
+    } else {
+        t = __fadd_rn(t, __fmul_rn(v1, F[3 + j]));
+        t = __fadd_rn(t, __fmul_rn(v2, F[6 + j]));
+    }
+    return t;
+}
+
+// Gather the value(s) landing on destination (x, y, z) for rotation F.
+//   KIND_VEC:     svec {vx, vy, vz, data != 0}        -> out[0..2]
+//   KIND_SCALAR:  sscal {channel value, data != 0}    -> out[0]
+//   KIND_PROB:    svec as above + sprob {p1..p4} each masked by its own != 0
+//                                                     -> out[0..6]
+//   KIND_DATA64:  sdat (float64, masked by != 0)      -> outd
+template <int KIND>
+__device__ __forceinline__ void gather_voxel(int x, int y, int z, int n, const float* F, bool fma,
+                                             const float4* __restrict__ svec,
+                                             const float2* __restrict__ sscal,
+                                             const float4* __restrict__ sprob,
+                                             const double* __restrict__ sdat,
+                                             float* out, double* outd) {
+    constexpr int NOUT = (KIND == KIND_VEC) ? 3 : (KIND == KIND_PROB ? 7 : 1);
+#pragma unroll
+    for (int c = 0; c < NOUT; ++c) out[c] = 0.f;
+    *outd = 0.0;
+    int cx[2], cy[2], cz[2];
+    const int nx = dest_candidates(x, n, cx);
+    const int ny = dest_candidates(y, n, cy);
+    const int nz = dest_candidates(z, n, cz);
+    const float cent = 0.5f * (float)n;
+    const float fn = (float)n;
+    bool done_v = false;
+    bool done_p[4] = {false, false, false, false};
+    for (int a = 0; a < nz; ++a)
+    for (int b = 0; b < nx; ++b)
+    for (int c = 0; c < ny; ++c) {
+        const float n0 = (float)cx[b] - cent, n1 = (float)cy[c] - cent, n2 = (float)cz[a] - cent;
+        const float o0 = rot_coord(n0, n1, n2, F + 0, cent, fma);
+        const float o1 = rot_coord(n0, n1, n2, F + 3, cent, fma);
+        const float o2 = rot_coord(n0, n1, n2, F + 6, cent, fma);
+        if (!(o0 >= 0.f && o1 >= 0.f && o2 >= 0.f && o0 < fn && o1 < fn && o2 < fn)) continue;
+        const size_t si = ((size_t)(int)o0 * n + (int)o1) * n + (int)o2;
+        if (KIND == KIND_VEC || KIND == KIND_PROB) {
+            if (!done_v) {
+                const float4 s = svec[si];
+                if (s.w != 0.f) {
+                    out[0] = rot_vec(s.x, s.y, s.z, F, 0, fma);
+                    out[1] = rot_vec(s.x, s.y, s.z, F, 1, fma);
+                    out[2] = rot_vec(s.x, s.y, s.z, F, 2, fma);
+                    done_v = true;
+                }
+            }
+            if (KIND == KIND_PROB) {
+                const float4 p = sprob[si];
+                const float pv[4] = {p.x, p.y, p.z, p.w};
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (!done_p[k] && pv[k] != 0.f) { out[3 + k] = pv[k]; done_p[k] = true; }
+                if (done_v && done_p[0] && done_p[1] && done_p[2] && done_p[3]) return;
+            } else if (done_v) {
+                return;
+            }
+        } else if (KIND == KIND_SCALAR) {
+            const float2 s = sscal[si];
+            if (s.y != 0.f) { out[0] = s.x; return; }
+        } else {
+            const double s = sdat[si];
+            if (s != 0.0) { *outd = s; return; }
+        }
+    }
+}
+
+template <int KIND>
+__global__ void __launch_bounds__(256)
+k_gather(const float4* __restrict__ svec, const float2* __restrict__ sscal,
+         const float4* __restrict__ sprob, const double* __restrict__ sdat,
+         const float* __restrict__ rot, int n, int nzb, int fma,
+         float* __restrict__ q, double* __restrict__ qd,
+         unsigned long long* __restrict__ keys, int nkeys) {
+    const int b = blockIdx.z;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < nkeys) keys[(size_t)b * nkeys + threadIdx.x] = 0ull;
+    const int x = blockIdx.y;
+    const int y = (blockIdx.x / nzb) * 4 + (threadIdx.x >> 6);
+    const int z = (blockIdx.x % nzb) * 64 + (threadIdx.x & 63);
+    if (y >= n || z >= n) return;
+    float F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = rot[b * 9 + i];
+    constexpr int NOUT = (KIND == KIND_VEC) ? 3 : (KIND == KIND_PROB ? 7 : 1);
+    float out[NOUT];
+    double outd;
+    gather_voxel<KIND>(x, y, z, n, F, fma != 0, svec, sscal, sprob, sdat, out, &outd);
+    const size_t n3 = (size_t)n * n * n;
+    const size_t v = ((size_t)x * n + y) * n + z;
+    if (KIND == KIND_DATA64) {
+        qd[(size_t)b * n3 + v] = outd;
+    } else {
+#pragma unroll
+        for (int c = 0; c < NOUT; ++c) q[((size_t)b * NOUT + c) * n3 + v] = out[c];
+    }
+}
+
+// Laplacian of gathered float64 data -> float32 query channel
+// (laplacian_filter, VESPER_1_prob_new.py:494-526, then np.copyto to float32).
+__global__ void k_lap_to_q(const double* __restrict__ qd, float* __restrict__ q, int n) {
+    const size_t n3 = (size_t)n * n * n;
+    const int b = blockIdx.y;
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n3) return;
+    const double* a = qd + (size_t)b * n3;
+    const int z = (int)(v % n), y = (int)((v / n) % n), x = (int)(v / ((size_t)n * n));
+    double r = 0.0;
+    const double c = a[v];
+    if (c > 0) {
+        r = -6.0 * c;
+        if (x + 1 < n) r += a[v + (size_t)n * n];
+        if (x - 1 >= 0) r += a[v - (size_t)n * n];
+        if (y + 1 < n) r += a[v + n];
+        if (y - 1 >= 0) r += a[v - n];
+        if (z + 1 < n) r += a[v + 1];
+        if (z - 1 >= 0) r += a[v - 1];
+    }
+    q[(size_t)b * n3 + v] = (float)r;
+}
+
+// ---------------------------------------------------------------------------
+// Source preparation
+// ---------------------------------------------------------------------------
+
+__global__ void k_build_src(const double* __restrict__ data, const float* __restrict__ vec,
+                            const float* __restrict__ prob, size_t n3, int mode, double ave,
+                            float4* __restrict__ svec, float2* __restrict__ sscal,
+                            float4* __restrict__ sprob) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const double d = data[i];
+    const float mask = (d != 0.0) ? 1.f : 0.f;
+    if (mode == VX_MODE_VEC || mode == VX_MODE_PROB) {
+        svec[i] = make_float4(vec[3 * i], vec[3 * i + 1], vec[3 * i + 2], mask);
+        if (mode == VX_MODE_PROB)
+            sprob[i] = make_float4(prob[i], prob[n3 + i], prob[2 * n3 + i], prob[3 * n3 + i]);
+    } else {
+        // query channel value per mode (VESPER_1_prob_new.py:548-553), cast
+        // to float32 as np.copyto into the float32 FFT input does
+        float v = 0.f;
+        if (d > 0) {
+            if (mode == VX_MODE_OVERLAP) v = 1.f;
+            else if (mode == VX_MODE_CC) v = (float)d;
+            else if (mode == VX_MODE_PCC) v = (float)(d - ave);
+        }
+        sscal[i] = make_float2(v, mask);
+    }
+}
+
+// Target channels (VESPER_1_prob_new.py:601-615, :831-838) as float32 planes.
+__global__ void k_target_chan(const double* __restrict__ data, const float* __restrict__ vec,
+                              const float* __restrict__ prob, const double* __restrict__ lap,
+                              size_t n3, int mode, double ave, float* __restrict__ q) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    if (mode == VX_MODE_VEC || mode == VX_MODE_PROB) {
+        q[i] = vec[3 * i];
+        q[n3 + i] = vec[3 * i + 1];
+        q[2 * n3 + i] = vec[3 * i + 2];
+        if (mode == VX_MODE_PROB)
+            for (int k = 0; k < 4; ++k) q[(3 + k) * n3 + i] = prob[k * n3 + i];
+        return;
+    }
+    const double d = data[i];
+    float v = 0.f;
+    if (mode == VX_MODE_LAPLACIAN) v = (float)lap[i];
+    else if (d > 0) {
+        if (mode == VX_MODE_OVERLAP) v = 1.f;
+        else if (mode == VX_MODE_CC) v = (float)d;
+        else if (mode == VX_MODE_PCC) v = (float)(d - ave);
+    }
+    q[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// FFT passes
+// ---------------------------------------------------------------------------
+
+// Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
+template <int MAXV>
+__global__ void __launch_bounds__(1024)
+k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int n, int nhz) {
+    extern __shared__ float2 lds[];
+    const int x = blockIdx.x;
+    const size_t plane = blockIdx.y;
+    const float* src = q + (plane * n + x) * (size_t)n * n;
+    const int npair = (n + 1) >> 1;
+    const int nthr = blockDim.x;
+    for (int e = threadIdx.x; e < npair * n; e += nthr) {
+        const int l = e / n, z = e - l * n;
+        const float re = src[(size_t)(2 * l) * n + z];
+        const float im = (2 * l + 1 < n) ? src[(size_t)(2 * l + 1) * n + z] : 0.f;
+        lds[e] = make_float2(re, im);
+    }
+    __syncthreads();
+    fft_lds<MAXV>(lds, fd, npair, 1, n, false);
+    // unpack Z = A + iB: A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
+    {
+        constexpr int MU = (MAXV + 1) / 2;
+        float2 A[MU], B[MU];
+        int dst[MU];
+        const int nu = npair * nhz;
+#pragma unroll
+        for (int m = 0; m < MU; ++m) {
+            const int u = threadIdx.x + m * nthr;
+            dst[m] = -1;
+            if (u < nu) {
+                const int l = u / nhz, k = u - l * nhz;
+                const float2 zk = lds[l * n + k];
+                const float2 w = lds[l * n + (k == 0 ? 0 : n - k)];
+                A[m] = make_float2(0.5f * (zk.x + w.x), 0.5f * (zk.y - w.y));
+                B[m] = make_float2(0.5f * (zk.y + w.y), 0.5f * (w.x - zk.x));
+                dst[m] = (2 * l) * nhz + k;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MU; ++m) {
+            if (dst[m] >= 0) {
+                lds[dst[m]] = A[m];
+                if (dst[m] + nhz < n * nhz) lds[dst[m] + nhz] = B[m];
+            }
+        }
+        __syncthreads();
+    }
+    fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);
+    float2* out = spec + (plane * n + x) * (size_t)n * nhz;
+    for (int e = threadIdx.x; e < n * nhz; e += nthr) out[e] = lds[e];
+}
+
+// x-axis: per tile of T flattened (ky, kz) columns.  For each channel c:
+// FFT along x, multiply by tgt[c] (already conjugated), accumulate into its
+// group; then inverse FFT along x of each group.
+//   MODE_FWD_CONJ (tgt == nullptr): forward only, write conj(result) per
+//   channel (target preparation).
+template <int MAXV, int G>
+__global__ void __launch_bounds__(256)
+k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+        FftDesc fd, int n, int nhz, int C, int T, unsigned grp_mask) {
+    extern __shared__ float2 lds[];
+    const int plane = n * nhz;
+    const int j0 = blockIdx.x * T;
+    const int nt = min(T, plane - j0);
+    const int b = blockIdx.y;
+    const int nthr = blockDim.x;
+    const int ne = n * nt;
+    float2 acc[G][MAXV];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) acc[g][m] = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; ++c) {
+        const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane;
+        for (int e = threadIdx.x; e < ne; e += nthr) {
+            const int xx = e / nt, t = e - xx * nt;
+            lds[e] = src[(size_t)xx * plane + j0 + t];
+        }
+        __syncthreads();
+        fft_lds<MAXV>(lds, fd, nt, nt, 1, true);
+        if (tgt == nullptr) {
+            float2* dst = out + ((size_t)b * C + c) * n * (size_t)plane;
+            for (int e = threadIdx.x; e < ne; e += nthr) {
+                const int xx = e / nt, t = e - xx * nt;
+                dst[(size_t)xx * plane + j0 + t] = cconj(lds[e]);
+            }
+            __syncthreads();
+            continue;
+        }
+        const float2* tc = tgt + (size_t)c * n * plane;
+        const int g = (grp_mask >> c) & 1;
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            if (e < ne) {
+                const int xx = e / nt, t = e - xx * nt;
+                const float2 p = cmul(lds[e], tc[(size_t)xx * plane + j0 + t]);
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg)
+                    if (gg == g) acc[gg][m] = cadd(acc[gg][m], p);
+            }
+        }
+        __syncthreads();
+    }
+    if (tgt == nullptr) return;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            if (e < ne) lds[e] = cconj(acc[g][m]);
+        }
+        __syncthreads();
+        fft_lds<MAXV>(lds, fd, nt, nt, 1, true);
+        float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane;
+        for (int e = threadIdx.x; e < ne; e += nthr) {
+            const int xx = e / nt, t = e - xx * nt;
+            dst[(size_t)xx * plane + j0 + t] = cconj(lds[e]);
+        }
+        __syncthreads();
+    }
+}
+
+// Inverse: sum[bg][x] ([ky][kz]) -> real slab ([y][z]) -> argmax key.
+// Optionally writes the real correlation (prob-mode mixing).
+template <int MAXV>
+__global__ void __launch_bounds__(1024)
+k_inv_yz(const float2* __restrict__ sum, FftDesc fd, int n, int nhz, int G,
+         unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
+    extern __shared__ float2 lds[];
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    const int x = blockIdx.x;
+    const int bg = blockIdx.y;
+    const int nthr = blockDim.x;
+    const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
+    for (int e = threadIdx.x; e < n * nhz; e += nthr) lds[e] = cconj(src[e]);
+    __syncthreads();
+    fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);   // conj of the y-inverse
+    const int npair = (n + 1) >> 1;
+    const int half = n >> 1;
+    const bool even = (n & 1) == 0;
+    // pack conj(Z), Z[k] = A[k] + i B[k] with A, B = rows 2l, 2l+1 after the
+    // y-inverse (lds holds r = conj of them); Hermitian extension for k > n/2;
+    // imaginary parts of the DC (and Nyquist) bins dropped as c2r does.
+    {
+        constexpr int MU = MAXV;
+        float2 Zc[MU];
+        int dst[MU];
+        const int nu = npair * n;
+#pragma unroll
+        for (int m = 0; m < MU; ++m) {
+            const int u = threadIdx.x + m * nthr;
+            dst[m] = -1;
+            if (u < nu) {
+                const int l = u / n, k = u - l * n;
+                const bool hasb = (2 * l + 1) < n;
+                if (k <= half) {
+                    float2 ra = lds[(2 * l) * nhz + k];
+                    float2 rb = hasb ? lds[(2 * l + 1) * nhz + k] : make_float2(0.f, 0.f);
+                    if (k == 0 || (even && k == half)) { ra.y = 0.f; rb.y = 0.f; }
+                    Zc[m] = make_float2(ra.x + rb.y, ra.y - rb.x);
+                } else {
+                    const int kk = n - k;
+                    float2 ra = lds[(2 * l) * nhz + kk];
+                    float2 rb = hasb ? lds[(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+                    Zc[m] = make_float2(ra.x - rb.y, -ra.y - rb.x);
+                }
+                dst[m] = l * n + k;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MU; ++m)
+            if (dst[m] >= 0) lds[dst[m]] = Zc[m];
+        __syncthreads();
+    }
+    fft_lds<MAXV>(lds, fd, npair, 1, n, false);
+    // z[n] = conj(F): row 2l = F.x, row 2l+1 = -F.y
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    const size_t n3 = (size_t)n * n * n;
+    for (int e = threadIdx.x; e < npair * n; e += nthr) {
+        const int l = e / n, z = e - l * n;
+        const float2 f = lds[e];
+        const uint32_t ia = ((uint32_t)x * n + 2 * l) * n + z;
+        const float va = f.x;
+        if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
+        if (real_out) real_out[(size_t)bg * n3 + ia] = va;
+        if (2 * l + 1 < n) {
+            const uint32_t ib = ia + n;
+            const float vb = -f.y;
+            if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
+            if (real_out) real_out[(size_t)bg * n3 + ib] = vb;
+        }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) {
+        const int b = bg / G, g = bg - b * G;
+        atomicMax(&keys[(size_t)b * nkeys + g], argmax_key(bv, bi));
+    }
+}
+
+// Prob-mode mixed score (find_best_trans_mixed, VESPER_1_prob_new.py:1530-1545)
+// in float32 with float32-rounded constants, no FMA:
+//   mixed = fl(fl(a * fl(fl(v - vave) / vstd)) + fl(b * fl(fl(p - pave) / pstd)))
+__global__ void __launch_bounds__(256)
+k_mixed(const float* __restrict__ real, size_t n3, float vave, float vstd, float pave,
+        float pstd, float wa, float wb, unsigned long long* __restrict__ keys, int nkeys) {
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    const int b = blockIdx.y;
+    const float* rv = real + (size_t)b * 2 * n3;
+    const float* rp = rv + n3;
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3;
+         i += (size_t)gridDim.x * blockDim.x) {
+        const float sv = __fdiv_rn(__fsub_rn(rv[i], vave), vstd);
+        const float sp = __fdiv_rn(__fsub_rn(rp[i], pave), pstd);
+        const float m = __fadd_rn(__fmul_rn(wa, sv), __fmul_rn(wb, sp));
+        if (better(m, (uint32_t)i, bv, bi)) { bv = m; bi = (uint32_t)i; }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) atomicMax(&keys[(size_t)b * nkeys + 2], argmax_key(bv, bi));
+}
+
+// keys [nb][nkeys] -> best/trans rows [row0, row0 + nb), nout entries per row
+// (nout <= nkeys: prob pass 1 reports {vec, prob} only).
+__global__ void k_finalize(const unsigned long long* __restrict__ keys, int nb, int nkeys, int nout,
+                           int n, int row0, float* __restrict__ best, int32_t* __restrict__ trans) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb * nout) return;
+    const int b = i / nout, g = i - b * nout;
+    const unsigned long long k = keys[(size_t)b * nkeys + g];
+    const uint32_t idx = 0xffffffffu - (uint32_t)(k & 0xffffffffu);
+    const size_t o = (size_t)(row0 + b) * nout + g;
+    best[o] = unordered_bits((uint32_t)(k >> 32));
+    trans[o * 3 + 0] = (int32_t)(idx / ((uint32_t)n * n));
+    trans[o * 3 + 1] = (int32_t)((idx / n) % n);
+    trans[o * 3 + 2] = (int32_t)(idx % n);
+}
+
+// ---------------------------------------------------------------------------
+// Launch helpers
+// ---------------------------------------------------------------------------
+
+template <typename K>
+static int allow_lds(K kernel, size_t bytes) {
+    if (bytes > 65536) VX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return VX_OK;
+}
+
+// Slab kernels: choose threads and per-thread value budget so that every
+// phase fits: z pass (npair x n), unpack (npair x nhz pairs, MAXV/2 each),
+// y pass (nhz x n), inverse pack (npair x n).
+static void slab_shape(int n, int nhz, int* nthr, int* maxv) {
+    const int npair = (n + 1) / 2;
+    const int opts[3] = {8, 16, 32};
+    for (int t : {256, 512, 1024})
+        for (int mv : opts)
+            if (n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2)) {
+                *nthr = t; *maxv = mv; return;
+            }
+    *nthr = 1024; *maxv = 0;
+}
+
+int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    int nthr, maxv;
+    slab_shape(n, nhz, &nthr, &maxv);
+    const size_t lds = (size_t)n * nhz * sizeof(float2);
+    VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
+    dim3 grid(n, nplanes);
+    switch (maxv) {
+        case 8:  allow_lds(k_fwd_zy<8>, lds);  hipLaunchKernelGGL(k_fwd_zy<8>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
+        case 16: allow_lds(k_fwd_zy<16>, lds); hipLaunchKernelGGL(k_fwd_zy<16>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
+        default: allow_lds(k_fwd_zy<32>, lds); hipLaunchKernelGGL(k_fwd_zy<32>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
+    }
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
+                  unsigned long long* keys, int nkeys, float* real_out, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    int nthr, maxv;
+    slab_shape(n, nhz, &nthr, &maxv);
+    const size_t lds = (size_t)n * nhz * sizeof(float2);
+    VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
+    dim3 grid(n, nbg);
+    switch (maxv) {
+        case 8:  allow_lds(k_inv_yz<8>, lds);  hipLaunchKernelGGL(k_inv_yz<8>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
+        case 16: allow_lds(k_inv_yz<16>, lds); hipLaunchKernelGGL(k_inv_yz<16>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
+        default: allow_lds(k_inv_yz<32>, lds); hipLaunchKernelGGL(k_inv_yz<32>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
+    }
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+static int xcorr_T(int n) {
+    // n * T <= 256 threads * 16 values
+    int T = 4096 / n;
+    if (T > 64) T = 64;
+    if (T < 1) T = 1;
+    return T;
+}
+
+int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
+                 int nb, int C, int G, unsigned grp_mask, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    const int T = xcorr_T(n);
+    VX_REQUIRE(n * T <= 256 * 16, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
+    const size_t lds = (size_t)n * T * sizeof(float2);
+    dim3 grid((n * nhz + T - 1) / T, nb);
+    if (G == 1) hipLaunchKernelGGL((k_xcorr<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
+    else        hipLaunchKernelGGL((k_xcorr<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
+                  const double* sdat, const float* rot, int n, int nb, int fma, float* q,
+                  double* qd, unsigned long long* keys, int nkeys, hipStream_t st) {
+    const int nzb = (n + 63) / 64;
+    dim3 grid(nzb * ((n + 3) / 4), n, nb);
+    switch (kind) {
+        case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
+        case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
+        case KIND_PROB:   hipLaunchKernelGGL(k_gather<KIND_PROB>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
+        default:          hipLaunchKernelGGL(k_gather<KIND_DATA64>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
+    }
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+}  // namespace vx
+
+// ---------------------------------------------------------------------------
+// Plan C-ABI
+// ---------------------------------------------------------------------------
+
+using namespace vx;
+
+struct vx_plan {
+    int n, nhz, mode, batch, flags;
+    int C;        // query channels
+    int G;        // correlation groups
+    int nkeys;    // keys per rotation (G, +1 for the prob mixed score)
+    int kind;     // gather kind
+    FftDesc fd;
+    float2* twid;
+    float4* svec;
+    float2* sscal;
+    float4* sprob;
+    double* sdat;
+    float* q;
+    double* qd;
+    float2* spec;
+    float2* sum;
+    float2* tgt;
+    unsigned long long* keys;
+    float* real_out;
+    float* rot_stage;
+    bool have_target, have_search, mix;
+    double alpha, vstd, vave, pstd, pave;
+    size_t bytes;
+};
+
+static void plan_free(vx_plan* p) {
+    if (!p) return;
+    void* ptrs[] = {p->twid, p->svec, p->sscal, p->sprob, p->sdat, p->q, p->qd, p->spec,
+                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage};
+    for (void* x : ptrs)
+        if (x) (void)hipFree(x);
+    delete p;
+}
+
+#define PLAN_ALLOC(field, count)                                                        \
+    do {                                                                                \
+        const size_t b_ = (size_t)(count) * sizeof(*p->field);                          \
+        if (hipMalloc((void**)&p->field, b_) != hipSuccess) {                           \
+            plan_free(p);                                                               \
+            (void)hipGetLastError();                                                    \
+            set_error("device allocation of %zu bytes for " #field " failed", b_);      \
+            return VX_ERR_NOMEM;                                                        \
+        }                                                                               \
+        p->bytes += b_;                                                                 \
+    } while (0)
+
+extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** out) {
+    VX_REQUIRE(out != nullptr, VX_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    VX_REQUIRE(n >= 2 && n <= 1024, VX_ERR_ARG, "grid edge %d out of range", n);
+    VX_REQUIRE(mode >= VX_MODE_VEC && mode <= VX_MODE_PROB, VX_ERR_ARG, "bad mode %d", mode);
+    VX_REQUIRE(batch >= 1 && batch <= 4096, VX_ERR_ARG, "bad batch %d", batch);
+    const int nhz = n / 2 + 1;
+    VX_REQUIRE((size_t)n * nhz * sizeof(float2) <= kMaxLds, VX_ERR_UNSUPPORTED,
+               "grid edge %d: slab of %zu bytes exceeds LDS", n, (size_t)n * nhz * sizeof(float2));
+    vx_plan* p = new (std::nothrow) vx_plan();
+    VX_REQUIRE(p != nullptr, VX_ERR_NOMEM, "host allocation failed");
+    memset(p, 0, sizeof(*p));
+    p->n = n; p->nhz = nhz; p->mode = mode; p->batch = batch; p->flags = flags;
+    switch (mode) {
+        case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
+        case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
+        case VX_MODE_LAPLACIAN: p->C = 1; p->G = 1; p->nkeys = 1; p->kind = KIND_DATA64; break;
+        default:           p->C = 1; p->G = 1; p->nkeys = 1; p->kind = KIND_SCALAR; break;
+    }
+    int rc = fft_desc_create(n, &p->fd, &p->twid);
+    if (rc != VX_OK) { plan_free(p); return rc; }
+    const size_t n3 = (size_t)n * n * n;
+    const size_t nh = (size_t)n * n * nhz;
+    const int B = batch;
+    if (p->kind == KIND_VEC || p->kind == KIND_PROB) PLAN_ALLOC(svec, n3);
+    if (p->kind == KIND_PROB) PLAN_ALLOC(sprob, n3);
+    if (p->kind == KIND_SCALAR) PLAN_ALLOC(sscal, n3);
+    if (p->kind == KIND_DATA64) { PLAN_ALLOC(sdat, n3); PLAN_ALLOC(qd, (size_t)B * n3); }
+    PLAN_ALLOC(q, (size_t)std::max(B, 1) * p->C * n3);
+    PLAN_ALLOC(spec, (size_t)B * p->C * nh);
+    PLAN_ALLOC(sum, (size_t)B * p->G * nh);
+    PLAN_ALLOC(tgt, (size_t)p->C * nh);
+    PLAN_ALLOC(keys, (size_t)B * p->nkeys);
+    PLAN_ALLOC(rot_stage, (size_t)9 * B);
+    if (mode == VX_MODE_PROB) PLAN_ALLOC(real_out, (size_t)B * 2 * n3);
+    *out = p;
+    return VX_OK;
+}
+
+extern "C" int vx_plan_destroy(vx_plan* p) {
+    if (p) {
+        (void)hipDeviceSynchronize();
+        plan_free(p);
+    }
+    return VX_OK;
+}
+
+extern "C" int vx_plan_info(const vx_plan* p, int64_t* info) {
+    VX_REQUIRE(p && info, VX_ERR_ARG, "NULL argument");
+    info[0] = p->n; info[1] = p->mode; info[2] = p->batch;
+    info[3] = p->mix ? 3 : p->nkeys == 3 ? 2 : p->G;
+    info[4] = p->C; info[5] = (int64_t)p->bytes; info[6] = 1; info[7] = 0;
+    return VX_OK;
+}
+
+extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec, const float* sprob,
+                             double t_ave, vx_stream stream) {
+    VX_REQUIRE(p && sdata, VX_ERR_ARG, "NULL plan or data");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n3 = (size_t)p->n * p->n * p->n;
+    if (p->kind == KIND_VEC || p->kind == KIND_PROB) VX_REQUIRE(svec, VX_ERR_ARG, "vec required");
+    if (p->kind == KIND_PROB) VX_REQUIRE(sprob, VX_ERR_ARG, "prob channels required");
+    if (p->kind == KIND_DATA64) {
+        VX_HIP(hipMemcpyAsync(p->sdat, sdata, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
+    } else {
+        hipLaunchKernelGGL(k_build_src, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                           sdata, svec, sprob, n3, p->mode, t_ave, p->svec, p->sscal, p->sprob);
+        VX_LAUNCH_CHECK();
+    }
+    p->have_search = true;
+    return VX_OK;
+}
+
+extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec, const float* tprob,
+                             double t_ave, vx_stream stream) {
+    VX_REQUIRE(p && tdata, VX_ERR_ARG, "NULL plan or data");
+    if (p->mode == VX_MODE_VEC || p->mode == VX_MODE_PROB) VX_REQUIRE(tvec, VX_ERR_ARG, "vec required");
+    if (p->mode == VX_MODE_PROB) VX_REQUIRE(tprob, VX_ERR_ARG, "prob channels required");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = p->n;
+    const size_t n3 = (size_t)n * n * n;
+    double* lap = nullptr;
+    if (p->mode == VX_MODE_LAPLACIAN) {
+        lap = p->qd;   // batch >= 1 float64 planes available
+        int rc = vx_laplacian(n, tdata, lap, stream);
+        if (rc != VX_OK) return rc;
+    }
+    hipLaunchKernelGGL(k_target_chan, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
+                       tdata, tvec, tprob, lap, n3, p->mode, t_ave, p->q);
+    VX_LAUNCH_CHECK();
+    // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
+    // spec holds B*C planes >= C planes.
+    int rc = launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, st);
+    if (rc != VX_OK) return rc;
+    // treat the C channels as "batch" 1 with C channels, forward-conj mode
+    rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, st);
+    if (rc != VX_OK) return rc;
+    p->have_target = true;
+    return VX_OK;
+}
+
+extern "C" int vx_set_prob_stats(vx_plan* p, int enable, double alpha, double vstd, double vave,
+                                 double pstd, double pave) {
+    VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
+    VX_REQUIRE(p->mode == VX_MODE_PROB, VX_ERR_STATE, "plan is not in prob mode");
+    p->mix = enable != 0;
+    p->alpha = alpha; p->vstd = vstd; p->vave = vave; p->pstd = pstd; p->pave = pave;
+    return VX_OK;
+}
+
+// Stage boundaries for vx_score_rows_profile: 0 gather, 1 fwd_zy, 2 xcorr,
+// 3 inv_yz, 4 mixed, 5 finalize.
+constexpr int kStages = 6;
+
+static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
+                           hipStream_t st, hipEvent_t* ev, double* stage_ms) {
+    VX_REQUIRE(p && rot && best && trans, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
+    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+    const int n = p->n;
+    const size_t n3 = (size_t)n * n * n;
+    const int nk = p->nkeys;
+    const int nout = (p->mode == VX_MODE_PROB) ? (p->mix ? 3 : 2) : 1;
+    const unsigned grp_mask = (p->mode == VX_MODE_PROB) ? 0x78u : 0u;
+#define VX_MARK(i) do { if (ev) VX_HIP(hipEventRecord(ev[i], st)); } while (0)
+    for (int r0 = 0; r0 < nrows; r0 += p->batch) {
+        const int nb = std::min(p->batch, nrows - r0);
+        VX_MARK(0);
+        int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot + (size_t)r0 * 9, n, nb,
+                               (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, p->q, p->qd, p->keys, nk, st);
+        if (rc != VX_OK) return rc;
+        if (p->kind == KIND_DATA64) {
+            hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), nb), dim3(256), 0, st, p->qd, p->q, n);
+            VX_LAUNCH_CHECK();
+        }
+        VX_MARK(1);
+        rc = launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, st);
+        if (rc != VX_OK) return rc;
+        VX_MARK(2);
+        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, st);
+        if (rc != VX_OK) return rc;
+        VX_MARK(3);
+        rc = launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk,
+                           (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr, st);
+        if (rc != VX_OK) return rc;
+        VX_MARK(4);
+        if (p->mode == VX_MODE_PROB && p->mix) {
+            const float wa = (float)(1.0 - p->alpha), wb = (float)p->alpha;
+            unsigned gx = (unsigned)std::min<size_t>((n3 + 255) / 256, 512);
+            hipLaunchKernelGGL(k_mixed, dim3(gx, nb), dim3(256), 0, st, p->real_out, n3,
+                               (float)p->vave, (float)p->vstd, (float)p->pave, (float)p->pstd, wa, wb,
+                               p->keys, nk);
+            VX_LAUNCH_CHECK();
+        }
+        VX_MARK(5);
+        hipLaunchKernelGGL(k_finalize, dim3((nb * nout + 255) / 256), dim3(256), 0, st, p->keys, nb, nk, nout,
+                           n, r0, best, trans);
+        VX_LAUNCH_CHECK();
+        VX_MARK(6);
+        if (ev) {
+            VX_HIP(hipEventSynchronize(ev[kStages]));
+            for (int i = 0; i < kStages; ++i) {
+                float ms = 0.f;
+                VX_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                stage_ms[i] += ms;
+            }
+        }
+    }
+#undef VX_MARK
+    return VX_OK;
+}
+
+extern "C" int vx_score_rows(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
+                             vx_stream stream) {
+    return score_rows_impl(p, rot, nrows, best, trans, (hipStream_t)stream, nullptr, nullptr);
+}
+
+extern "C" int vx_score_rows_profile(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
+                                     double* stage_ms, vx_stream stream) {
+    VX_REQUIRE(stage_ms, VX_ERR_ARG, "NULL stage_ms");
+    hipEvent_t ev[kStages + 1];
+    for (int i = 0; i <= kStages; ++i) VX_HIP(hipEventCreate(&ev[i]));
+    for (int i = 0; i < kStages; ++i) stage_ms[i] = 0.0;
+    const int rc = score_rows_impl(p, rot, nrows, best, trans, (hipStream_t)stream, ev, stage_ms);
+    for (int i = 0; i <= kStages; ++i) (void)hipEventDestroy(ev[i]);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// Single-rotation map rotation (rot_mrc / rot_mrc_prob API)
+// ---------------------------------------------------------------------------
+
+namespace vx {
+
+__global__ void __launch_bounds__(256)
+k_rotate_map(int n, int nzb, RotF rf, int fma, const double* __restrict__ data, const float* __restrict__ vec,
+             const float* __restrict__ prob, double* __restrict__ odata, float* __restrict__ ovec,
+             float* __restrict__ oprob) {
+    const int x = blockIdx.y;
+    const int y = (blockIdx.x / nzb) * 4 + (threadIdx.x >> 6);
+    const int z = (blockIdx.x % nzb) * 64 + (threadIdx.x & 63);
+    if (y >= n || z >= n) return;
+    const size_t n3 = (size_t)n * n * n;
+    const float* F = rf.m;
+    int cx[2], cy[2], cz[2];
+    const int nx = dest_candidates(x, n, cx);
+    const int ny = dest_candidates(y, n, cy);
+    const int nz = dest_candidates(z, n, cz);
+    const float cent = 0.5f * (float)n;
+    const float fn = (float)n;
+    double od = 0.0;
+    float ov[3] = {0.f, 0.f, 0.f};
+    float op[4] = {0.f, 0.f, 0.f, 0.f};
+    bool done_v = false;
+    bool done_p[4] = {prob == nullptr, prob == nullptr, prob == nullptr, prob == nullptr};
+    for (int a = 0; a < nz; ++a)
+    for (int b = 0; b < nx; ++b)
+    for (int c = 0; c < ny; ++c) {
+        const float n0 = (float)cx[b] - cent, n1 = (float)cy[c] - cent, n2 = (float)cz[a] - cent;
+        const float o0 = rot_coord(n0, n1, n2, F + 0, cent, fma != 0);
+        const float o1 = rot_coord(n0, n1, n2, F + 3, cent, fma != 0);
+        const float o2 = rot_coord(n0, n1, n2, F + 6, cent, fma != 0);
+        if (!(o0 >= 0.f && o1 >= 0.f && o2 >= 0.f && o0 < fn && o1 < fn && o2 < fn)) continue;
+        const size_t si = ((size_t)(int)o0 * n + (int)o1) * n + (int)o2;
+        if (!done_v && data[si] != 0.0) {
+            od = data[si];
+            const float s0 = vec[3 * si], s1 = vec[3 * si + 1], s2 = vec[3 * si + 2];
+            ov[0] = rot_vec(s0, s1, s2, F, 0, fma != 0);
+            ov[1] = rot_vec(s0, s1, s2, F, 1, fma != 0);
+            ov[2] = rot_vec(s0, s1, s2, F, 2, fma != 0);
+            done_v = true;
+        }
+        for (int k = 0; k < 4; ++k) {
+            if (!done_p[k]) {
+                const float pv = prob[k * n3 + si];
+                if (pv != 0.f) { op[k] = pv; done_p[k] = true; }
+            }
+        }
+        if (done_v && done_p[0] && done_p[1] && done_p[2] && done_p[3]) break;
+    }
+    const size_t v = ((size_t)x * n + y) * n + z;
+    odata[v] = od;
+    ovec[3 * v] = ov[0];
+    ovec[3 * v + 1] = ov[1];
+    ovec[3 * v + 2] = ov[2];
+    if (oprob)
+        for (int k = 0; k < 4; ++k) oprob[k * n3 + v] = op[k];
+}
+
+}  // namespace vx
+
+extern "C" int vx_rotate_map(int n, const float* rot9, const double* data, const float* vec, const float* prob,
+                             double* odata, float* ovec, float* oprob, int flags, vx_stream stream) {
+    VX_REQUIRE(n > 0 && rot9 && data && vec && odata && ovec, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE((prob == nullptr) == (oprob == nullptr), VX_ERR_ARG, "prob and out_prob must both be set or NULL");
+    RotF rf;
+    memcpy(rf.m, rot9, sizeof(rf.m));
+    const int nzb = (n + 63) / 64;
+    dim3 grid(nzb * ((n + 3) / 4), n);
+    hipLaunchKernelGGL(k_rotate_map, grid, dim3(256), 0, (hipStream_t)stream, n, nzb, rf,
+                       (flags & VX_FLAG_NO_FMA) ? 0 : 1, data, vec, prob, odata, ovec, oprob);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}

@@ -1,0 +1,182 @@
+"""Device-side API over libvesper_hip: device maps, search plans, fastVEC,
+map rotation, FP64 scoring.  PyTorch-ROCm tensors are used only as device
+allocations and for the current HIP stream."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _native as nat
+from ._native import VesperError
+
+
+def require_gpu() -> torch.device:
+    if not torch.cuda.is_available():
+        raise VesperError("no ROCm GPU visible: vesper_python_amd runs on MI355X only "
+                          "(there is no CPU fallback)")
+    nat.load()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _p(t) -> ctypes.c_void_p:
+    return ctypes.c_void_p(0 if t is None else t.data_ptr())
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def to_dev(a, dtype) -> torch.Tensor:
+    dev = require_gpu()
+    if isinstance(a, torch.Tensor):
+        return a.to(device=dev, dtype=dtype).contiguous()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
+
+
+_TORCH_OF = {np.float32: torch.float32, np.float64: torch.float64}
+
+
+def _host3(v) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(v, dtype=np.float64).reshape(3))
+
+
+def fastvec(src_data, src_width, src_orig, n, dest_width, dest_orig, dreso=16.0):
+    """Gaussian-weighted vector field of a thresholded source map
+    (doFastVEC, VESPER_1_prob_new.py:180-257).  Returns device tensors
+    data [n,n,n] float64 and vec [n,n,n,3] float32."""
+    src = to_dev(np.asarray(src_data, dtype=np.float32), torch.float32)
+    sx, sy, sz = (int(s) for s in src.shape)
+    data = torch.empty((n, n, n), dtype=torch.float64, device=src.device)
+    vec = torch.empty((n, n, n, 3), dtype=torch.float32, device=src.device)
+    so = _host3(np.asarray(src_orig, dtype=np.float32))
+    do = _host3(dest_orig)
+    nat.call("vx_fastvec", _p(src), sx, sy, sz, float(np.float32(src_width)),
+             so.ctypes.data_as(ctypes.c_void_p), int(n), float(dest_width),
+             do.ctypes.data_as(ctypes.c_void_p), float(dreso), _p(data), _p(vec), _stream())
+    return data, vec
+
+
+def map_stats(data: torch.Tensor) -> dict:
+    """Statistics of VESPER_1_prob_new.py:157-175 computed on the device."""
+    out = np.zeros(5, dtype=np.float64)
+    d = data.contiguous()
+    nat.call("vx_map_stats", _p(d), int(d.numel()), out.ctypes.data_as(ctypes.c_void_p), _stream())
+    return dict(dsum=out[0], Nact=int(out[1]), ave=out[2], std=out[3], std_norm_ave=out[4])
+
+
+def laplacian(data: torch.Tensor) -> torch.Tensor:
+    out = torch.empty_like(data)
+    nat.call("vx_laplacian", int(data.shape[0]), _p(data), _p(out), _stream())
+    return out
+
+
+def rotate_map(rot9, data: torch.Tensor, vec: torch.Tensor, prob: torch.Tensor | None = None,
+               flags: int = 0):
+    """rot_mrc / rot_mrc_prob for one rotation (VESPER_1_prob_new.py:279-354,
+    :1211-1299).  rot9 = float32(flip(M)) row-major."""
+    n = int(data.shape[0])
+    r = np.ascontiguousarray(np.asarray(rot9, dtype=np.float32).reshape(9))
+    od = torch.empty_like(data)
+    ov = torch.empty_like(vec)
+    op = torch.empty_like(prob) if prob is not None else None
+    nat.call("vx_rotate_map", n, r.ctypes.data_as(ctypes.c_void_p), _p(data), _p(vec), _p(prob),
+             _p(od), _p(ov), _p(op), int(flags), _stream())
+    return od, ov, op
+
+
+def get_score(tdata, tvec, t_ave, sdata, svec, trans) -> dict:
+    """get_score sums in FP64 (VESPER_1_prob_new.py:1302-1393)."""
+    n = int(tdata.shape[0])
+    t = np.ascontiguousarray(np.asarray(trans, dtype=np.int32).reshape(3))
+    out = np.zeros(10, dtype=np.float64)
+    nat.call("vx_get_score", n, _p(tdata), _p(tvec), float(t_ave), _p(sdata), _p(svec),
+             t.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), _stream())
+    return dict(Nm=int(out[0]), total=int(out[1]), cc_raw=out[2], pcc_raw=out[3], score=out[4],
+                ave2=out[5], std2=out[6], pstd2=out[7])
+
+
+def topn(scores: torch.Tensor, k: int, stride: int = 1, col: int = 0) -> np.ndarray:
+    """Rows of the k best scores, (score desc, row asc)."""
+    nrows = scores.numel() // stride
+    k_eff = min(int(k), nrows)
+    rows = torch.empty(max(k_eff, 1), dtype=torch.int32, device=scores.device)
+    nat.call("vx_topn", _p(scores), int(nrows), int(stride), int(col), int(max(k, 1)), _p(rows), _stream())
+    return rows[:k_eff].cpu().numpy()
+
+
+class SearchPlan:
+    """A vx_plan: target spectra + query source for one (N, mode) search."""
+
+    def __init__(self, n: int, mode: int, batch: int = 8, flags: int = 0):
+        require_gpu()
+        h = ctypes.c_void_p()
+        nat.call("vx_plan_create", int(n), int(mode), int(batch), int(flags), ctypes.byref(h))
+        self._h = h
+        self.n, self.mode, self.batch, self.flags = int(n), int(mode), int(batch), int(flags)
+        self._keep = []
+        self.mix = False
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            nat.load().vx_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def info(self) -> dict:
+        out = np.zeros(8, dtype=np.int64)
+        nat.call("vx_plan_info", self._h, out.ctypes.data_as(ctypes.c_void_p))
+        return dict(n=int(out[0]), mode=int(out[1]), batch=int(out[2]), groups=int(out[3]),
+                    channels=int(out[4]), workspace_bytes=int(out[5]))
+
+    def set_target(self, data, vec, prob=None, ave=0.0):
+        nat.call("vx_set_target", self._h, _p(data), _p(vec), _p(prob), float(ave), _stream())
+
+    def set_search(self, data, vec, prob=None, ave=0.0):
+        self._keep = [data, vec, prob]
+        nat.call("vx_set_search", self._h, _p(data), _p(vec), _p(prob), float(ave), _stream())
+
+    def set_prob_stats(self, alpha, vstd, vave, pstd, pave, enable=True):
+        self.mix = bool(enable)
+        nat.call("vx_set_prob_stats", self._h, int(bool(enable)), float(alpha), float(vstd),
+                 float(vave), float(pstd), float(pave))
+
+    @property
+    def groups(self) -> int:
+        if self.mode == nat.VX_MODE_PROB:
+            return 3 if self.mix else 2
+        return 1
+
+    def score_rows(self, rot: torch.Tensor, best: torch.Tensor | None = None,
+                   trans: torch.Tensor | None = None):
+        """Score every row of rot [R, 9] (device float32).  Returns (best [R, G]
+        float32, trans [R, G, 3] int32) device tensors."""
+        R = int(rot.shape[0])
+        G = self.groups
+        if best is None:
+            best = torch.empty((R, G), dtype=torch.float32, device=rot.device)
+        if trans is None:
+            trans = torch.empty((R, G, 3), dtype=torch.int32, device=rot.device)
+        if R:
+            nat.call("vx_score_rows", self._h, _p(rot), R, _p(best), _p(trans), _stream())
+        return best, trans
+
+    STAGES = ("gather", "fwd_zy", "xcorr", "inv_yz", "mixed", "finalize")
+
+    def profile_rows(self, rot: torch.Tensor) -> dict:
+        """Per-stage milliseconds (HIP events on the launch stream) of scoring
+        every row of rot, summed over batches."""
+        R = int(rot.shape[0])
+        G = self.groups
+        best = torch.empty((R, G), dtype=torch.float32, device=rot.device)
+        trans = torch.empty((R, G, 3), dtype=torch.int32, device=rot.device)
+        ms = np.zeros(6, dtype=np.float64)
+        nat.call("vx_score_rows_profile", self._h, _p(rot), R, _p(best), _p(trans),
+                 ms.ctypes.data_as(ctypes.c_void_p), _stream())
+        return dict(zip(self.STAGES, ms.tolist()))
