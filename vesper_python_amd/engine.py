@@ -28,14 +28,15 @@ def _stream() -> ctypes.c_void_p:
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+_NUMPY_OF = {torch.float32: np.float32, torch.float64: np.float64, torch.int32: np.int32}
+
+
 def to_dev(a, dtype) -> torch.Tensor:
+    """Copy a host array (or tensor) to the GPU as a contiguous `dtype` tensor."""
     dev = require_gpu()
     if isinstance(a, torch.Tensor):
         return a.to(device=dev, dtype=dtype).contiguous()
-    return torch.from_numpy(np.ascontiguousarray(a, dtype=dtype)).to(dev)
-
-
-_TORCH_OF = {np.float32: torch.float32, np.float64: torch.float64}
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=_NUMPY_OF[dtype])).to(dev)
 
 
 def _host3(v) -> np.ndarray:
@@ -46,7 +47,8 @@ def fastvec(src_data, src_width, src_orig, n, dest_width, dest_orig, dreso=16.0)
     """Gaussian-weighted vector field of a thresholded source map
     (doFastVEC, VESPER_1_prob_new.py:180-257).  Returns device tensors
     data [n,n,n] float64 and vec [n,n,n,3] float32."""
-    src = to_dev(np.asarray(src_data, dtype=np.float32), torch.float32)
+    src = to_dev(src_data if isinstance(src_data, torch.Tensor) else np.asarray(src_data, dtype=np.float32),
+                 torch.float32)
     sx, sy, sz = (int(s) for s in src.shape)
     data = torch.empty((n, n, n), dtype=torch.float64, device=src.device)
     vec = torch.empty((n, n, n, 3), dtype=torch.float32, device=src.device)
