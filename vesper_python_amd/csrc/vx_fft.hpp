@@ -23,14 +23,31 @@ namespace vx {
 
 constexpr int kMaxStages = 16;
 
+// Per-stage parameters live in device memory (read with uniform scalar
+// loads); only the pointers and sizes travel as kernel arguments.
+struct FftStage {
+    int R;          // radix
+    int P;          // product of the earlier radices
+    float Pinv;     // 1 / P, for the in-kernel k = i mod P
+    int tw;         // float2 offset of the stage's twiddles
+};
+
 struct FftDesc {
     int n;
     int nst;
-    int R[kMaxStages];
-    int P[kMaxStages];
-    int tw[kMaxStages];          // float2 offset of each stage's twiddles
-    const float2* twid;          // device table
+    const FftStage* st;          // device [nst]
+    const float2* twid;          // device twiddle table
 };
+
+// Quotient of small non-negative integers (a < 2^22, d < 2^16) through a
+// float reciprocal, corrected to exact: avoids the ~40-instruction integer
+// division in butterfly index math.
+__device__ __forceinline__ int fdiv(int a, int d, float dinv) {
+    int q = (int)((float)a * dinv);
+    const int r = a - q * d;
+    q += (r >= d) ? 1 : ((r < 0) ? -1 : 0);
+    return q;
+}
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -153,11 +170,12 @@ __device__ __forceinline__ void dft(float2 (&x)[R]) {
 //              line stride is 1) instead of consecutive butterflies.
 template <int R, int MAXV>
 __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es, int ls,
-                                          bool line_fast, int p, const float2* __restrict__ tw) {
+                                          bool line_fast, int p, float pinv, const float2* __restrict__ tw) {
     constexpr int MAXB = (MAXV + R - 1) / R;
     const int T = n / R;
     const int nb = T * nlines;
     const int nthr = blockDim.x;
+    const float linv = 1.0f / (float)nlines, tinv = 1.0f / (float)T;
     float2 v[MAXB][R];
     int obase[MAXB];
 #pragma unroll
@@ -166,12 +184,12 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
         obase[m] = -1;
         if (b < nb) {
             int line, i;
-            if (line_fast) { i = b / nlines; line = b - i * nlines; }
-            else           { line = b / T;   i = b - line * T; }
+            if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
+            else           { line = fdiv(b, T, tinv);   i = b - line * T; }
             float2* L = buf + line * ls;
 #pragma unroll
             for (int j = 0; j < R; ++j) v[m][j] = L[(i + j * T) * es];
-            const int k = (p == 1) ? 0 : (i % p);
+            const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
             if (k != 0) {
                 const float2* w = tw + k * (R - 1);
 #pragma unroll
@@ -197,19 +215,78 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
 template <int MAXV>
 __device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int ls, bool line_fast) {
     for (int s = 0; s < d.nst; ++s) {
-        const float2* tw = d.twid + d.tw[s];
-        switch (d.R[s]) {
-            case 2:  fft_stage<2, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 3:  fft_stage<3, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 4:  fft_stage<4, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 5:  fft_stage<5, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 7:  fft_stage<7, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 8:  fft_stage<8, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 11: fft_stage<11, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
-            case 13: fft_stage<13, MAXV>(buf, d.n, nlines, es, ls, line_fast, d.P[s], tw); break;
+        const FftStage sg = d.st[s];
+        const float2* tw = d.twid + sg.tw;
+        const int p = sg.P;
+        const float pi = sg.Pinv;
+        switch (sg.R) {
+            case 2:  fft_stage<2, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 3:  fft_stage<3, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 4:  fft_stage<4, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 5:  fft_stage<5, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 7:  fft_stage<7, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 8:  fft_stage<8, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 11: fft_stage<11, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 13: fft_stage<13, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
             default: break;
         }
     }
+}
+
+// Ping-pong variant: each stage reads `in` and writes `out` (two LDS
+// buffers), so a thread keeps only one butterfly in registers at a time.
+// Ends synchronised; returns the buffer that holds the result.
+template <int R>
+__device__ __forceinline__ void fft_stage_pp(const float2* __restrict__ in, float2* __restrict__ out, int n,
+                                             int nlines, int es, int ls, bool line_fast, int p, float pinv,
+                                             const float2* __restrict__ tw) {
+    const int T = n / R;
+    const int nb = T * nlines;
+    const float linv = 1.0f / (float)nlines, tinv = 1.0f / (float)T;
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+        int line, i;
+        if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
+        else           { line = fdiv(b, T, tinv);   i = b - line * T; }
+        const float2* L = in + line * ls;
+        float2 v[R];
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = L[(i + j * T) * es];
+        const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
+        if (k != 0) {
+            const float2* w = tw + k * (R - 1);
+#pragma unroll
+            for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[j - 1]);
+        }
+        dft<R>(v);
+        float2* O = out + line * ls + ((i - k) * R + k) * es;
+        const int pes = p * es;
+#pragma unroll
+        for (int j = 0; j < R; ++j) O[j * pes] = v[j];
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ float2* fft_lds_pp(float2* a, float2* b, const FftDesc& d, int nlines, int es, int ls,
+                                              bool line_fast) {
+    for (int s = 0; s < d.nst; ++s) {
+        const FftStage sg = d.st[s];
+        const float2* tw = d.twid + sg.tw;
+        const int p = sg.P;
+        const float pi = sg.Pinv;
+        switch (sg.R) {
+            case 2:  fft_stage_pp<2>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 3:  fft_stage_pp<3>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 4:  fft_stage_pp<4>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 5:  fft_stage_pp<5>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 7:  fft_stage_pp<7>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 8:  fft_stage_pp<8>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 11: fft_stage_pp<11>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            case 13: fft_stage_pp<13>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            default: break;
+        }
+        float2* t = a; a = b; b = t;
+    }
+    return a;
 }
 
 }  // namespace vx

@@ -54,13 +54,15 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
     memset(d, 0, sizeof(*d));
     d->n = n;
     d->nst = (int)r.size();
+    std::vector<FftStage> stages(d->nst);
     std::vector<float2> tw;
     int p = 1;
     for (int s = 0; s < d->nst; ++s) {
         const int R = r[s];
-        d->R[s] = R;
-        d->P[s] = p;
-        d->tw[s] = (int)tw.size();
+        stages[s].R = R;
+        stages[s].P = p;
+        stages[s].Pinv = 1.0f / (float)p;
+        stages[s].tw = (int)tw.size();
         for (int k = 0; k < p; ++k)
             for (int j = 1; j < R; ++j) {
                 const double a = -2.0 * M_PI * (double)j * (double)k / ((double)p * (double)R);
@@ -69,11 +71,15 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
         p *= R;
     }
     if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
-    float2* dev = nullptr;
-    VX_HIP(hipMalloc(&dev, tw.size() * sizeof(float2)));
-    VX_HIP(hipMemcpy(dev, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
-    d->twid = dev;
-    *twid_out = dev;
+    // one allocation: stage table (padded to 256 B) then twiddles
+    const size_t sbytes = 256;
+    char* dev = nullptr;
+    VX_HIP(hipMalloc(&dev, sbytes + tw.size() * sizeof(float2)));
+    VX_HIP(hipMemcpy(dev, stages.data(), stages.size() * sizeof(FftStage), hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(dev + sbytes, tw.data(), tw.size() * sizeof(float2), hipMemcpyHostToDevice));
+    d->st = (const FftStage*)dev;
+    d->twid = (const float2*)(dev + sbytes);
+    *twid_out = (float2*)dev;
     return VX_OK;
 }
 
@@ -296,8 +302,8 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // ---------------------------------------------------------------------------
 
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
-template <int MAXV>
-__global__ void __launch_bounds__(1024)
+template <int MAXV, int NT>
+__global__ void __launch_bounds__(NT)
 k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int n, int nhz) {
     extern __shared__ float2 lds[];
     const int x = blockIdx.x;
@@ -305,11 +311,26 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
     const float* src = q + (plane * n + x) * (size_t)n * n;
     const int npair = (n + 1) >> 1;
     const int nthr = blockDim.x;
-    for (int e = threadIdx.x; e < npair * n; e += nthr) {
-        const int l = e / n, z = e - l * n;
-        const float re = src[(size_t)(2 * l) * n + z];
-        const float im = (2 * l + 1 < n) ? src[(size_t)(2 * l + 1) * n + z] : 0.f;
-        lds[e] = make_float2(re, im);
+    {
+        // all loads of the slab in flight before the first LDS write
+        const float ninv = 1.0f / (float)n;
+        float re[MAXV], im[MAXV];
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            re[m] = 0.f;
+            im[m] = 0.f;
+            if (e < npair * n) {
+                const int l = fdiv(e, n, ninv), z = e - l * n;
+                re[m] = src[(size_t)(2 * l) * n + z];
+                if (2 * l + 1 < n) im[m] = src[(size_t)(2 * l + 1) * n + z];
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            if (e < npair * n) lds[e] = make_float2(re[m], im[m]);
+        }
     }
     __syncthreads();
     fft_lds<MAXV>(lds, fd, npair, 1, n, false);
@@ -344,7 +365,11 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
     }
     fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);
     float2* out = spec + (plane * n + x) * (size_t)n * nhz;
-    for (int e = threadIdx.x; e < n * nhz; e += nthr) out[e] = lds[e];
+#pragma unroll
+    for (int m = 0; m < MAXV; ++m) {
+        const int e = threadIdx.x + m * nthr;
+        if (e < n * nhz) out[e] = lds[e];
+    }
 }
 
 // x-axis: per tile of T flattened (ky, kz) columns.  For each channel c:
@@ -419,10 +444,96 @@ k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2*
     }
 }
 
+// x pass, correlate mode: one workgroup per (tile of T flattened (ky, kz)
+// columns, rotation b).  For each channel: FFT along x in two ping-pong LDS
+// buffers, times the conjugated target tile, accumulated per group in
+// registers; then the inverse FFT along x of each group.  Block ids are
+// laid out so that the B workgroups of one tile share an XCD (id mod 8) and
+// are dispatched together: the target tile is read from HBM once per batch
+// and from that XCD's L2 by the other rotations.
+template <int MAXE, int G>
+__global__ void __launch_bounds__(256)
+k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+           FftDesc fd, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask) {
+    extern __shared__ float2 lds[];
+    float2* A = lds;
+    float2* Bf = lds + n * T;
+    const int id = blockIdx.x;
+    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
+    const int b = rem >> 3;
+    const int tile = grp * 8 + (rem & 7);
+    if (tile >= ntiles) return;
+    const int plane = n * nhz;
+    const int j0 = tile * T;
+    const int nt = min(T, plane - j0);
+    const int ne = n * nt;
+    const float ntinv = 1.0f / (float)nt;
+    float2 acc[G][MAXE];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int m = 0; m < MAXE; ++m) acc[g][m] = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; ++c) {
+        const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane + j0;
+        {
+            float2 v[MAXE];
+#pragma unroll
+            for (int m = 0; m < MAXE; ++m) {
+                const int e = threadIdx.x + m * 256;
+                if (e < ne) {
+                    const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+                    v[m] = src[(size_t)xx * plane + t];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < MAXE; ++m) {
+                const int e = threadIdx.x + m * 256;
+                if (e < ne) A[e] = v[m];
+            }
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+        const float2* tc = tgt + (size_t)c * n * plane + j0;
+        const int g = (grp_mask >> c) & 1;
+#pragma unroll
+        for (int m = 0; m < MAXE; ++m) {
+            const int e = threadIdx.x + m * 256;
+            if (e < ne) {
+                const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+                const float2 pr = cmul(r[e], tc[(size_t)xx * plane + t]);
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg)
+                    if (gg == g) acc[gg][m] = cadd(acc[gg][m], pr);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int m = 0; m < MAXE; ++m) {
+            const int e = threadIdx.x + m * 256;
+            if (e < ne) A[e] = cconj(acc[g][m]);
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+        float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
+#pragma unroll
+        for (int m = 0; m < MAXE; ++m) {
+            const int e = threadIdx.x + m * 256;
+            if (e < ne) {
+                const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+                dst[(size_t)xx * plane + t] = cconj(r[e]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // Inverse: sum[bg][x] ([ky][kz]) -> real slab ([y][z]) -> argmax key.
 // Optionally writes the real correlation (prob-mode mixing).
-template <int MAXV>
-__global__ void __launch_bounds__(1024)
+template <int MAXV, int NT>
+__global__ void __launch_bounds__(NT)
 k_inv_yz(const float2* __restrict__ sum, FftDesc fd, int n, int nhz, int G,
          unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
     extern __shared__ float2 lds[];
@@ -432,7 +543,19 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fd, int n, int nhz, int G,
     const int bg = blockIdx.y;
     const int nthr = blockDim.x;
     const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
-    for (int e = threadIdx.x; e < n * nhz; e += nthr) lds[e] = cconj(src[e]);
+    {
+        float2 v[MAXV];
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            if (e < n * nhz) v[m] = src[e];
+        }
+#pragma unroll
+        for (int m = 0; m < MAXV; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            if (e < n * nhz) lds[e] = cconj(v[m]);
+        }
+    }
     __syncthreads();
     fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);   // conj of the y-inverse
     const int npair = (n + 1) >> 1;
@@ -551,17 +674,25 @@ static int allow_lds(K kernel, size_t bytes) {
 
 // Slab kernels: choose threads and per-thread value budget so that every
 // phase fits: z pass (npair x n), unpack (npair x nhz pairs, MAXV/2 each),
-// y pass (nhz x n), inverse pack (npair x n).
+// y pass (nhz x n), inverse pack (npair x n).  Variants (MAXV, threads):
+// (8, 512), (16, 512), (32, 512), (24, 1024).
 static void slab_shape(int n, int nhz, int* nthr, int* maxv) {
     const int npair = (n + 1) / 2;
-    const int opts[3] = {8, 16, 32};
-    for (int t : {256, 512, 1024})
-        for (int mv : opts)
-            if (n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2)) {
-                *nthr = t; *maxv = mv; return;
-            }
+    const int opts[4][2] = {{8, 512}, {16, 512}, {32, 512}, {24, 1024}};
+    for (auto& o : opts) {
+        const int mv = o[0], t = o[1];
+        if (n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2)) {
+            *nthr = t; *maxv = mv; return;
+        }
+    }
     *nthr = 1024; *maxv = 0;
 }
+
+#define VX_SLAB_LAUNCH(KER, MV, NTH, ...)                                              \
+    do {                                                                               \
+        allow_lds(KER<MV, NTH>, lds);                                                  \
+        hipLaunchKernelGGL((KER<MV, NTH>), grid, dim3(NTH), lds, st, __VA_ARGS__);     \
+    } while (0)
 
 int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, hipStream_t st) {
     const int nhz = n / 2 + 1;
@@ -570,11 +701,10 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     const size_t lds = (size_t)n * nhz * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nplanes);
-    switch (maxv) {
-        case 8:  allow_lds(k_fwd_zy<8>, lds);  hipLaunchKernelGGL(k_fwd_zy<8>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
-        case 16: allow_lds(k_fwd_zy<16>, lds); hipLaunchKernelGGL(k_fwd_zy<16>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
-        default: allow_lds(k_fwd_zy<32>, lds); hipLaunchKernelGGL(k_fwd_zy<32>, grid, dim3(nthr), lds, st, q, spec, fd, n, nhz); break;
-    }
+    if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz);
+    else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz);
+    else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz);
+    else                 VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -587,11 +717,10 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     const size_t lds = (size_t)n * nhz * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
-    switch (maxv) {
-        case 8:  allow_lds(k_inv_yz<8>, lds);  hipLaunchKernelGGL(k_inv_yz<8>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
-        case 16: allow_lds(k_inv_yz<16>, lds); hipLaunchKernelGGL(k_inv_yz<16>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
-        default: allow_lds(k_inv_yz<32>, lds); hipLaunchKernelGGL(k_inv_yz<32>, grid, dim3(nthr), lds, st, sum, fd, n, nhz, G, keys, nkeys, real_out); break;
-    }
+    if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else                 VX_SLAB_LAUNCH(k_inv_yz, 24, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -604,15 +733,35 @@ static int xcorr_T(int n) {
     return T;
 }
 
+static int xcorr_pp_T(int n) {
+    // n * T <= 256 threads * 16 values; T <= 16 keeps two tiles of LDS small
+    int T = 4096 / n;
+    if (T > 16) T = 16;
+    if (T < 1) T = 1;
+    return T;
+}
+
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
                  int nb, int C, int G, unsigned grp_mask, hipStream_t st) {
     const int nhz = n / 2 + 1;
-    const int T = xcorr_T(n);
+    if (tgt == nullptr) {
+        // target preparation: forward x FFT + conj per channel (once per search)
+        const int T = xcorr_T(n);
+        VX_REQUIRE(n * T <= 256 * 16, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
+        const size_t lds = (size_t)n * T * sizeof(float2);
+        dim3 grid((n * nhz + T - 1) / T, nb);
+        hipLaunchKernelGGL((k_xcorr<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
+        VX_LAUNCH_CHECK();
+        return VX_OK;
+    }
+    const int T = xcorr_pp_T(n);
     VX_REQUIRE(n * T <= 256 * 16, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
-    const size_t lds = (size_t)n * T * sizeof(float2);
-    dim3 grid((n * nhz + T - 1) / T, nb);
-    if (G == 1) hipLaunchKernelGGL((k_xcorr<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
-    else        hipLaunchKernelGGL((k_xcorr<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
+    const size_t lds = 2 * (size_t)n * T * sizeof(float2);
+    const int ntiles = (n * nhz + T - 1) / T;
+    const int groups8 = (ntiles + 7) / 8;
+    dim3 grid((unsigned)(groups8 * 8 * nb));
+    if (G == 1) hipLaunchKernelGGL((k_xcorr_pp<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask);
+    else        hipLaunchKernelGGL((k_xcorr_pp<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
