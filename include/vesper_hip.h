@@ -49,7 +49,8 @@ extern "C" {
 #define VX_MODE_PROB 5
 
 /* flags */
-#define VX_FLAG_NO_FMA 1 /* K=3 rotation products rounded separately (no FMA) */
+#define VX_FLAG_NO_FMA 1   /* K=3 rotation products rounded separately (no FMA) */
+#define VX_FLAG_NO_PRUNE 2 /* transform the whole grid instead of the query's support box */
 
 typedef struct vx_plan vx_plan;
 typedef void* vx_stream; /* hipStream_t */
@@ -89,7 +90,7 @@ int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** out);
 int vx_plan_destroy(vx_plan* plan);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,
- *                 slab kernels fused (1/0), 0} */
+ *                 support box start, support box width} */
 int vx_plan_info(const vx_plan* plan, int64_t* info_host);
 
 /*

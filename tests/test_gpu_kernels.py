@@ -214,3 +214,37 @@ def test_get_score_fp64(gpu, trans):
     assert np.isclose(r["score"], e["score"], rtol=1e-12)
     e32 = ref.get_score(t, sd, sv, trans, "ref32")
     assert abs(r["score"] - e32["score"]) <= 1e-5 * max(1.0, abs(e32["score"]))
+
+
+@pytest.mark.parametrize("n", [24, 25, 40])
+def test_support_box_pruning_is_exact(gpu, n):
+    """A query whose density sits in a small central ball: the pruned search
+    (gather and forward FFT restricted to the support box) returns bit-for-bit
+    the unpruned results, and both match the oracle."""
+    rng = np.random.default_rng(500 + n)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.6)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    outside = np.linalg.norm(g, axis=-1) > n / 5
+    sd[outside] = 0
+    sv[outside] = 0
+    angles = np.concatenate([[[0, 0, 0]], rng.uniform(0, 360, (7, 3))])
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    res = []
+    for flags in (0, nat.VX_FLAG_NO_PRUNE):
+        _say("prune", n, flags)
+        plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=3, flags=flags)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+        info = plan.info()
+        if flags == 0:
+            assert info["box_width"] < n
+        else:
+            assert info["box_lo"] == 0 and info["box_width"] == n
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    o32, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref64")
+    _check_rows(res[0][0][:, 0], res[0][1][:, 0], o32, o64, f"prune n={n}")

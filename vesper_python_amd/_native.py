@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(_PKG, "libvesper_hip.so")
 
 VX_MODE_VEC, VX_MODE_OVERLAP, VX_MODE_CC, VX_MODE_PCC, VX_MODE_LAPLACIAN, VX_MODE_PROB = range(6)
 VX_FLAG_NO_FMA = 1
+VX_FLAG_NO_PRUNE = 2
 ABI_VERSION = 1
 
 MODE_CODES = {"VecProduct": VX_MODE_VEC, "Overlap": VX_MODE_OVERLAP, "CC": VX_MODE_CC,

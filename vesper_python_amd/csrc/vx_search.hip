@@ -196,13 +196,15 @@ k_gather(const float4* __restrict__ svec, const float2* __restrict__ sscal,
          const float4* __restrict__ sprob, const double* __restrict__ sdat,
          const float* __restrict__ rot, int n, int nzb, int fma,
          float* __restrict__ q, double* __restrict__ qd,
-         unsigned long long* __restrict__ keys, int nkeys) {
+         unsigned long long* __restrict__ keys, int nkeys, int lo, int w) {
     const int b = blockIdx.z;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < nkeys) keys[(size_t)b * nkeys + threadIdx.x] = 0ull;
-    const int x = blockIdx.y;
-    const int y = (blockIdx.x / nzb) * 4 + (threadIdx.x >> 6);
-    const int z = (blockIdx.x % nzb) * 64 + (threadIdx.x & 63);
-    if (y >= n || z >= n) return;
+    // destination box [lo, lo + w)^3 holds every voxel that can receive a
+    // non-zero value; the planes are zero outside it
+    const int x = lo + blockIdx.y;
+    const int y = lo + (blockIdx.x / nzb) * 4 + (threadIdx.x >> 6);
+    const int z = lo + (blockIdx.x % nzb) * 64 + (threadIdx.x & 63);
+    if (y >= lo + w || z >= lo + w) return;
     float F[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) F[i] = rot[b * 9 + i];
@@ -304,12 +306,16 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
 template <int MAXV, int NT>
 __global__ void __launch_bounds__(NT)
-k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int n, int nhz) {
+k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int n, int nhz, int lo, int w) {
+    // slabs x in [lo, lo + w) only; rows y outside [lo, lo + w) are zero, so
+    // only the line pairs [l0, l1) covering them are transformed along z
     extern __shared__ float2 lds[];
-    const int x = blockIdx.x;
+    const int x = lo + blockIdx.x;
     const size_t plane = blockIdx.y;
     const float* src = q + (plane * n + x) * (size_t)n * n;
     const int npair = (n + 1) >> 1;
+    const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
+    const int nact = (l1 - l0) * n;
     const int nthr = blockDim.x;
     {
         // all loads of the slab in flight before the first LDS write
@@ -320,8 +326,8 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
             const int e = threadIdx.x + m * nthr;
             re[m] = 0.f;
             im[m] = 0.f;
-            if (e < npair * n) {
-                const int l = fdiv(e, n, ninv), z = e - l * n;
+            if (e < nact) {
+                const int l = l0 + fdiv(e, n, ninv), z = e - (l - l0) * n;
                 re[m] = src[(size_t)(2 * l) * n + z];
                 if (2 * l + 1 < n) im[m] = src[(size_t)(2 * l + 1) * n + z];
             }
@@ -329,11 +335,11 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
 #pragma unroll
         for (int m = 0; m < MAXV; ++m) {
             const int e = threadIdx.x + m * nthr;
-            if (e < npair * n) lds[e] = make_float2(re[m], im[m]);
+            if (e < nact) lds[l0 * n + e] = make_float2(re[m], im[m]);
         }
     }
     __syncthreads();
-    fft_lds<MAXV>(lds, fd, npair, 1, n, false);
+    fft_lds<MAXV>(lds + l0 * n, fd, l1 - l0, 1, n, false);
     // unpack Z = A + iB: A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
     {
         constexpr int MU = (MAXV + 1) / 2;
@@ -346,10 +352,14 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
             dst[m] = -1;
             if (u < nu) {
                 const int l = u / nhz, k = u - l * nhz;
-                const float2 zk = lds[l * n + k];
-                const float2 w = lds[l * n + (k == 0 ? 0 : n - k)];
-                A[m] = make_float2(0.5f * (zk.x + w.x), 0.5f * (zk.y - w.y));
-                B[m] = make_float2(0.5f * (zk.y + w.y), 0.5f * (w.x - zk.x));
+                A[m] = make_float2(0.f, 0.f);
+                B[m] = make_float2(0.f, 0.f);
+                if (l >= l0 && l < l1) {
+                    const float2 zk = lds[l * n + k];
+                    const float2 wv = lds[l * n + (k == 0 ? 0 : n - k)];
+                    A[m] = make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
+                    B[m] = make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x));
+                }
                 dst[m] = (2 * l) * nhz + k;
             }
         }
@@ -454,7 +464,7 @@ k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2*
 template <int MAXE, int G>
 __global__ void __launch_bounds__(256)
 k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
-           FftDesc fd, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask) {
+           FftDesc fd, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
     extern __shared__ float2 lds[];
     float2* A = lds;
     float2* Bf = lds + n * T;
@@ -482,7 +492,8 @@ k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
                 const int e = threadIdx.x + m * 256;
                 if (e < ne) {
                     const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
-                    v[m] = src[(size_t)xx * plane + t];
+                    // slabs outside [lo, lo + w) of the query spectra are zero
+                    v[m] = (xx >= lo && xx < lo + w) ? src[(size_t)xx * plane + t] : make_float2(0.f, 0.f);
                 }
             }
 #pragma unroll
@@ -694,17 +705,18 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv) {
         hipLaunchKernelGGL((KER<MV, NTH>), grid, dim3(NTH), lds, st, __VA_ARGS__);     \
     } while (0)
 
-int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, hipStream_t st) {
+int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                  hipStream_t st) {
     const int nhz = n / 2 + 1;
     int nthr, maxv;
     slab_shape(n, nhz, &nthr, &maxv);
     const size_t lds = (size_t)n * nhz * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
-    dim3 grid(n, nplanes);
-    if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz);
-    else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz);
-    else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz);
-    else                 VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz);
+    dim3 grid(w, nplanes);
+    if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
+    else                 VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz, lo, w);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -742,7 +754,7 @@ static int xcorr_pp_T(int n) {
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
-                 int nb, int C, int G, unsigned grp_mask, hipStream_t st) {
+                 int nb, int C, int G, unsigned grp_mask, int lo, int w, hipStream_t st) {
     const int nhz = n / 2 + 1;
     if (tgt == nullptr) {
         // target preparation: forward x FFT + conj per channel (once per search)
@@ -760,22 +772,22 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
     const int ntiles = (n * nhz + T - 1) / T;
     const int groups8 = (ntiles + 7) / 8;
     dim3 grid((unsigned)(groups8 * 8 * nb));
-    if (G == 1) hipLaunchKernelGGL((k_xcorr_pp<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask);
-    else        hipLaunchKernelGGL((k_xcorr_pp<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask);
+    if (G == 1) hipLaunchKernelGGL((k_xcorr_pp<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    else        hipLaunchKernelGGL((k_xcorr_pp<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
 
 int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
                   const double* sdat, const float* rot, int n, int nb, int fma, float* q,
-                  double* qd, unsigned long long* keys, int nkeys, hipStream_t st) {
-    const int nzb = (n + 63) / 64;
-    dim3 grid(nzb * ((n + 3) / 4), n, nb);
+                  double* qd, unsigned long long* keys, int nkeys, int lo, int w, hipStream_t st) {
+    const int nzb = (w + 63) / 64;
+    dim3 grid(nzb * ((w + 3) / 4), w, nb);
     switch (kind) {
-        case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
-        case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
-        case KIND_PROB:   hipLaunchKernelGGL(k_gather<KIND_PROB>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
-        default:          hipLaunchKernelGGL(k_gather<KIND_DATA64>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys); break;
+        case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
+        case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
+        case KIND_PROB:   hipLaunchKernelGGL(k_gather<KIND_PROB>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
+        default:          hipLaunchKernelGGL(k_gather<KIND_DATA64>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
     }
     VX_LAUNCH_CHECK();
     return VX_OK;
@@ -810,6 +822,8 @@ struct vx_plan {
     float* real_out;
     float* rot_stage;
     bool have_target, have_search, mix;
+    bool q_dirty;     // q / qd hold non-zero values outside the support box
+    int lo, w;        // support box [lo, lo + w)^3 of the rotated query
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
 };
@@ -848,6 +862,7 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     VX_REQUIRE(p != nullptr, VX_ERR_NOMEM, "host allocation failed");
     memset(p, 0, sizeof(*p));
     p->n = n; p->nhz = nhz; p->mode = mode; p->batch = batch; p->flags = flags;
+    p->lo = 0; p->w = n; p->q_dirty = true;
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
@@ -886,9 +901,32 @@ extern "C" int vx_plan_info(const vx_plan* p, int64_t* info) {
     VX_REQUIRE(p && info, VX_ERR_ARG, "NULL argument");
     info[0] = p->n; info[1] = p->mode; info[2] = p->batch;
     info[3] = p->mix ? 3 : p->nkeys == 3 ? 2 : p->G;
-    info[4] = p->C; info[5] = (int64_t)p->bytes; info[6] = 1; info[7] = 0;
+    info[4] = p->C; info[5] = (int64_t)p->bytes; info[6] = p->lo; info[7] = p->w;
     return VX_OK;
 }
+
+namespace vx {
+// Largest squared distance from the rotation centre n/2 of a voxel whose
+// density (or, in prob mode, any probability channel) is non-zero.
+__global__ void __launch_bounds__(256)
+k_support_r2(const double* __restrict__ data, const float* __restrict__ prob, int n,
+             unsigned long long* __restrict__ out) {
+    const size_t n3 = (size_t)n * n * n;
+    const double c = 0.5 * n;
+    double best = 0.0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        bool nz = data[i] != 0.0;
+        if (prob) nz = nz || prob[i] != 0.f || prob[n3 + i] != 0.f || prob[2 * n3 + i] != 0.f || prob[3 * n3 + i] != 0.f;
+        if (nz) {
+            const int z = (int)(i % n), y = (int)((i / n) % n), x = (int)(i / ((size_t)n * n));
+            const double dx = x - c, dy = y - c, dz = z - c;
+            best = fmax(best, dx * dx + dy * dy + dz * dz);
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) best = fmax(best, __shfl_xor(best, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(best));
+}
+}  // namespace vx
 
 extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec, const float* sprob,
                              double t_ave, vx_stream stream) {
@@ -903,6 +941,32 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
         hipLaunchKernelGGL(k_build_src, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
                            sdata, svec, sprob, n3, p->mode, t_ave, p->svec, p->sscal, p->sprob);
         VX_LAUNCH_CHECK();
+    }
+    // Support box of every rotation of the query: a destination voxel d takes
+    // source voxel floor(F (i - c) + c) with |i - d| <= 1 per axis (odd-N
+    // fold), so a non-zero value lands only within radius R_s + 2 sqrt(3) of c,
+    // R_s = the largest centre distance of a non-zero source voxel.
+    if (p->flags & VX_FLAG_NO_PRUNE) {
+        p->lo = 0;
+        p->w = p->n;
+    } else {
+        unsigned long long* d_r2 = p->keys;   // scratch: keys are rewritten by every batch
+        VX_HIP(hipMemsetAsync(d_r2, 0, sizeof(unsigned long long), st));
+        hipLaunchKernelGGL(k_support_r2, dim3(512), dim3(256), 0, st, sdata,
+                           p->kind == KIND_PROB ? sprob : nullptr, p->n, d_r2);
+        VX_LAUNCH_CHECK();
+        unsigned long long h = 0;
+        VX_HIP(hipMemcpyAsync(&h, d_r2, sizeof(h), hipMemcpyDeviceToHost, st));
+        VX_HIP(hipStreamSynchronize(st));
+        double r2;
+        memcpy(&r2, &h, sizeof(r2));
+        const double c = 0.5 * p->n;
+        const double r = sqrt(r2) + 2.0 * sqrt(3.0) + 1.0;
+        int lo = (int)floor(c - r), hi = (int)ceil(c + r);
+        if (lo < 0) lo = 0;
+        if (hi > p->n - 1) hi = p->n - 1;
+        p->lo = lo;
+        p->w = hi - lo + 1;
     }
     p->have_search = true;
     return VX_OK;
@@ -927,12 +991,13 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     VX_LAUNCH_CHECK();
     // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
     // spec holds B*C planes >= C planes.
-    int rc = launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, st);
+    int rc = launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st);
     if (rc != VX_OK) return rc;
     // treat the C channels as "batch" 1 with C channels, forward-conj mode
-    rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, st);
+    rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, 0, n, st);
     if (rc != VX_OK) return rc;
     p->have_target = true;
+    p->q_dirty = true;
     return VX_OK;
 }
 
@@ -959,22 +1024,28 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
     const int nk = p->nkeys;
     const int nout = (p->mode == VX_MODE_PROB) ? (p->mix ? 3 : 2) : 1;
     const unsigned grp_mask = (p->mode == VX_MODE_PROB) ? 0x78u : 0u;
+    if (p->q_dirty) {
+        // the gather writes only the support box: clear the planes once
+        VX_HIP(hipMemsetAsync(p->q, 0, (size_t)p->batch * p->C * n3 * sizeof(float), st));
+        if (p->qd) VX_HIP(hipMemsetAsync(p->qd, 0, (size_t)p->batch * n3 * sizeof(double), st));
+        p->q_dirty = false;
+    }
 #define VX_MARK(i) do { if (ev) VX_HIP(hipEventRecord(ev[i], st)); } while (0)
     for (int r0 = 0; r0 < nrows; r0 += p->batch) {
         const int nb = std::min(p->batch, nrows - r0);
         VX_MARK(0);
         int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot + (size_t)r0 * 9, n, nb,
-                               (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, p->q, p->qd, p->keys, nk, st);
+                               (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, p->q, p->qd, p->keys, nk, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         if (p->kind == KIND_DATA64) {
             hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), nb), dim3(256), 0, st, p->qd, p->q, n);
             VX_LAUNCH_CHECK();
         }
         VX_MARK(1);
-        rc = launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, st);
+        rc = launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         VX_MARK(2);
-        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, st);
+        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         VX_MARK(3);
         rc = launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk,

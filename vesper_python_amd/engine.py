@@ -135,7 +135,8 @@ class SearchPlan:
         out = np.zeros(8, dtype=np.int64)
         nat.call("vx_plan_info", self._h, out.ctypes.data_as(ctypes.c_void_p))
         return dict(n=int(out[0]), mode=int(out[1]), batch=int(out[2]), groups=int(out[3]),
-                    channels=int(out[4]), workspace_bytes=int(out[5]))
+                    channels=int(out[4]), workspace_bytes=int(out[5]), box_lo=int(out[6]),
+                    box_width=int(out[7]))
 
     def set_target(self, data, vec, prob=None, ave=0.0):
         nat.call("vx_set_target", self._h, _p(data), _p(vec), _p(prob), float(ave), _stream())
