@@ -37,14 +37,22 @@ WORKLOADS = {
 }
 
 
-def _stage_bytes(n: int) -> dict:
-    """Algorithmic HBM bytes per scored row and stage (V mode, 3 channels):
-    gather reads the float4 source once and writes 3 float32 planes; the
-    forward pass reads the planes and writes 3 half spectra; the x pass reads
-    3 query + 3 target spectra and writes 1; the inverse pass reads 1."""
-    n3 = n ** 3
-    nh = n * n * (n // 2 + 1)
-    return {"gather": 16 * n3 + 12 * n3, "fwd_zy": 12 * n3 + 24 * nh, "xcorr": 56 * nh, "inv_yz": 8 * nh}
+def _stage_bytes(n: int, w: int, b: int) -> dict:
+    """Algorithmic HBM bytes per launch (b rotations, V mode, 3 channels) of
+    each stage, with the query's support box [lo, lo + w)^3:
+      gather   reads the float4 source under the box once, writes 3 float32
+               planes inside the box;
+      fwd_zy   reads the w slabs' rows inside the box, writes 3 half spectra
+               of the w slabs;
+      xcorr    reads those spectra, the 3 target spectra once per launch,
+               writes 1 half spectrum per rotation;
+      inv_yz   reads that spectrum."""
+    nhz = n // 2 + 1
+    nh = n * n * nhz
+    return {"gather": b * (16 + 12) * w ** 3,
+            "fwd_zy": b * 3 * (w * (w + 1) * n * 4 + w * n * nhz * 8),
+            "xcorr": b * (3 * w * n * nhz * 8 + nh * 8) + 3 * nh * 8,
+            "inv_yz": b * nh * 8}
 
 
 def prepare(workload: str):
@@ -98,6 +106,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-rows", type=int, default=512)
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -121,6 +130,12 @@ def main():
     plan.set_target(tdata, tvec, None, n1.ave)
     plan.set_search(sdata, svec, None, n1.ave)
     rot = torch.from_numpy(rotation_rows(angles)).to(dev)
+    tuned = None
+    if not args.no_tune:
+        from vesper_python_amd import autotune
+        # strided sample of the real rows; the schedule choice is cached per (device, N, mode, batch)
+        tuned = autotune.tune(plan, rot[::max(1, R // 256)].contiguous(), sample_rows=128, use_cache=False,
+                              verbose=True)
     best = torch.empty((R, 1), dtype=torch.float32, device=dev)
     trans = torch.empty((R, 1, 3), dtype=torch.int32, device=dev)
     print(f"[bench] workload={args.workload} N={n} rows={R} batch={args.batch}", file=sys.stderr, flush=True)
@@ -151,15 +166,18 @@ def main():
     prows = min(R, max(args.batch, args.profile_rows // args.batch * args.batch))
     stage_ms = plan.profile_rows(rot[:prows])
     launches = (prows + args.batch - 1) // args.batch
-    sb = _stage_bytes(n)
+    info = plan.info()
+    sb = _stage_bytes(n, info["box_width"], args.batch)
     dom = max(sb, key=lambda k: stage_ms[k])
     avg_launch_s = stage_ms[dom] / launches / 1e3
-    achieved = sb[dom] * args.batch / avg_launch_s / 1e9
+    achieved = sb[dom] / avg_launch_s / 1e9
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                 "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                 "avg_launch_ms": round(stage_ms[dom] / launches, 4),
                 "stage_ms_per_launch": {k: round(v / launches, 4) for k, v in stage_ms.items()},
-                "bytes_per_row": sb[dom], "rows_per_launch": args.batch}
+                "bytes_per_launch": sb[dom], "rows_per_launch": args.batch,
+                "support_box": [info["box_lo"], info["box_width"]],
+                "stage_GBps": {k: round(sb[k] / (stage_ms[k] / launches / 1e3) / 1e9, 1) for k in sb}}
 
     cpu = None
     if not args.no_cpu:
@@ -169,7 +187,8 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic", "top10_recall": 1.0,
            "config": {"workload": f"{args.workload}: {w['text']}", "grid_edge": n, "rows": R,
-                      "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]]},
+                      "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
+                      "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)

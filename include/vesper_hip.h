@@ -89,6 +89,16 @@ int vx_map_stats(const double* data_dev, int64_t count, double* out5_host,
 int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** out);
 int vx_plan_destroy(vx_plan* plan);
 
+/*
+ * Kernel schedule of a plan (the autotuner's knobs; results do not depend on
+ * them).  VX_TUNE_XPASS_VARIANT: 0 = x pass accumulating channels in
+ * registers, 1 = in LDS.  VX_TUNE_XPASS_TILE: columns per x-pass tile
+ * (0 = default).
+ */
+#define VX_TUNE_XPASS_VARIANT 1
+#define VX_TUNE_XPASS_TILE 2
+int vx_plan_tune(vx_plan* plan, int key, int value);
+
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,
  *                 support box start, support box width} */
 int vx_plan_info(const vx_plan* plan, int64_t* info_host);

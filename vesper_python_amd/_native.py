@@ -15,6 +15,8 @@ LIB_PATH = os.path.join(_PKG, "libvesper_hip.so")
 VX_MODE_VEC, VX_MODE_OVERLAP, VX_MODE_CC, VX_MODE_PCC, VX_MODE_LAPLACIAN, VX_MODE_PROB = range(6)
 VX_FLAG_NO_FMA = 1
 VX_FLAG_NO_PRUNE = 2
+VX_TUNE_XPASS_VARIANT = 1
+VX_TUNE_XPASS_TILE = 2
 ABI_VERSION = 1
 
 MODE_CODES = {"VecProduct": VX_MODE_VEC, "Overlap": VX_MODE_OVERLAP, "CC": VX_MODE_CC,
@@ -33,6 +35,7 @@ SIGNATURES = {
     "vx_plan_create": (_I, [_I, _I, _I, _I, ctypes.POINTER(_P)]),
     "vx_plan_destroy": (_I, [_P]),
     "vx_plan_info": (_I, [_P, _P]),
+    "vx_plan_tune": (_I, [_P, _I, _I]),
     "vx_set_target": (_I, [_P, _P, _P, _P, _D, _P]),
     "vx_set_search": (_I, [_P, _P, _P, _P, _D, _P]),
     "vx_set_prob_stats": (_I, [_P, _I, _D, _D, _D, _D, _D]),

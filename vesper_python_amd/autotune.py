@@ -1,0 +1,91 @@
+"""Autotuner for the search kernels' schedules.
+
+For a plan (grid edge N, mode, batch), times every candidate schedule of the
+x pass (register- or LDS-accumulating kernel, tile width) on a sample of the
+actual rotation rows with the library's per-stage HIP-event profiler
+(vx_score_rows_profile), keeps the fastest, and caches it per
+(device, N, mode, batch) in a JSON file.  Every schedule computes identical
+results; only the time differs.
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import threading
+
+import torch
+
+CACHE = os.environ.get("VX_TUNE_CACHE",
+                       os.path.join(os.path.expanduser("~"), ".cache", "vesper_python_amd", "tune.json"))
+_lock = threading.Lock()
+
+
+def candidates(n: int):
+    out = [dict(xpass_variant=0, xpass_tile=0)]
+    for t in (2, 4, 8, 12, 16):
+        out.append(dict(xpass_variant=1, xpass_tile=t))
+    return out
+
+
+def _key(plan) -> str:
+    name = torch.cuda.get_device_name(torch.cuda.current_device()) if torch.cuda.is_available() else "?"
+    return f"{name}|n={plan.n}|mode={plan.mode}|batch={plan.batch}|groups={plan.groups}"
+
+
+def _load() -> dict:
+    try:
+        with open(CACHE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(d: dict) -> None:
+    try:
+        os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+        tmp = CACHE + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(d, f, indent=1, sort_keys=True)
+        os.replace(tmp, CACHE)
+    except OSError:
+        pass
+
+
+def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True, verbose: bool = False) -> dict:
+    """Pick the fastest schedule for `plan` on rows `rot` (device [R, 9]);
+    applies it and returns {"schedule": ..., "ms": {...}}."""
+    key = _key(plan)
+    with _lock:
+        cache = _load() if use_cache else {}
+    if use_cache and key in cache:
+        sched = cache[key]["schedule"]
+        plan.tune(**sched)
+        return dict(cache[key], cached=True)
+    nrows = min(int(rot.shape[0]), max(plan.batch, sample_rows // plan.batch * plan.batch))
+    sample = rot[:nrows]
+    results = []
+    for cand in candidates(plan.n):
+        try:
+            plan.tune(**cand)
+            plan.score_rows(sample[:plan.batch])          # warm-up of this schedule
+            ms = plan.profile_rows(sample)
+        except Exception as e:  # a schedule the device rejects is skipped
+            if verbose:
+                print("[tune] skip", cand, e, file=sys.stderr)
+            continue
+        total = sum(ms.values())
+        results.append((total, cand, ms))
+        if verbose:
+            print(f"[tune] {cand} total={total:.3f} ms xcorr={ms['xcorr']:.3f}", file=sys.stderr, flush=True)
+    results.sort(key=lambda r: r[0])
+    total, best, ms = results[0]
+    plan.tune(**best)
+    rec = {"schedule": best, "ms_per_row": total / nrows, "rows": nrows,
+           "tried": [{"schedule": c, "ms_per_row": t / nrows} for t, c, _ in results]}
+    if use_cache:
+        with _lock:
+            cache = _load()
+            cache[key] = rec
+            _store(cache)
+    return dict(rec, cached=False)

@@ -35,9 +35,19 @@ struct FftStage {
 struct FftDesc {
     int n;
     int nst;
+    int ntw;                     // twiddle table length (float2)
     const FftStage* st;          // device [nst]
-    const float2* twid;          // device twiddle table
+    const float2* twid;          // twiddle table: device, or LDS once staged
 };
+
+// Copy the twiddle table into LDS (all threads of the block take part; the
+// caller's next __syncthreads publishes it) and return a descriptor reading it.
+__device__ __forceinline__ FftDesc stage_twiddles(const FftDesc& d, float2* lds_tw) {
+    for (int i = threadIdx.x; i < d.ntw; i += blockDim.x) lds_tw[i] = d.twid[i];
+    FftDesc r = d;
+    r.twid = lds_tw;
+    return r;
+}
 
 // Quotient of small non-negative integers (a < 2^22, d < 2^16) through a
 // float reciprocal, corrected to exact: avoids the ~40-instruction integer
@@ -237,32 +247,39 @@ __device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int l
 // buffers), so a thread keeps only one butterfly in registers at a time.
 // Ends synchronised; returns the buffer that holds the result.
 template <int R>
+__device__ __forceinline__ void butterfly_pp(const float2* __restrict__ in, float2* __restrict__ out, int b, int T,
+                                             int nlines, float linv, float tinv, int es, int ls, bool line_fast,
+                                             int p, float pinv, const float2* __restrict__ tw) {
+    int line, i;
+    if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
+    else           { line = fdiv(b, T, tinv);   i = b - line * T; }
+    const float2* L = in + line * ls;
+    float2 v[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) v[j] = L[(i + j * T) * es];
+    const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
+    if (k != 0) {
+        const float2* w = tw + k * (R - 1);
+#pragma unroll
+        for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[j - 1]);
+    }
+    dft<R>(v);
+    float2* O = out + line * ls + ((i - k) * R + k) * es;
+    const int pes = p * es;
+#pragma unroll
+    for (int j = 0; j < R; ++j) O[j * pes] = v[j];
+}
+
+template <int R>
 __device__ __forceinline__ void fft_stage_pp(const float2* __restrict__ in, float2* __restrict__ out, int n,
                                              int nlines, int es, int ls, bool line_fast, int p, float pinv,
                                              const float2* __restrict__ tw) {
     const int T = n / R;
     const int nb = T * nlines;
+    const int nthr = blockDim.x;
     const float linv = 1.0f / (float)nlines, tinv = 1.0f / (float)T;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
-        int line, i;
-        if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
-        else           { line = fdiv(b, T, tinv);   i = b - line * T; }
-        const float2* L = in + line * ls;
-        float2 v[R];
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = L[(i + j * T) * es];
-        const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
-        if (k != 0) {
-            const float2* w = tw + k * (R - 1);
-#pragma unroll
-            for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[j - 1]);
-        }
-        dft<R>(v);
-        float2* O = out + line * ls + ((i - k) * R + k) * es;
-        const int pes = p * es;
-#pragma unroll
-        for (int j = 0; j < R; ++j) O[j * pes] = v[j];
-    }
+    for (int b = threadIdx.x; b < nb; b += nthr)
+        butterfly_pp<R>(in, out, b, T, nlines, linv, tinv, es, ls, line_fast, p, pinv, tw);
     __syncthreads();
 }
 

@@ -71,6 +71,7 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
         p *= R;
     }
     if (tw.empty()) tw.push_back(make_float2(1.f, 0.f));
+    d->ntw = (int)tw.size();
     // one allocation: stage table (padded to 256 B) then twiddles
     const size_t sbytes = 256;
     char* dev = nullptr;
@@ -306,10 +307,11 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
 template <int MAXV, int NT>
 __global__ void __launch_bounds__(NT)
-k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int n, int nhz, int lo, int w) {
+k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w) {
     // slabs x in [lo, lo + w) only; rows y outside [lo, lo + w) are zero, so
     // only the line pairs [l0, l1) covering them are transformed along z
     extern __shared__ float2 lds[];
+    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
     const int x = lo + blockIdx.x;
     const size_t plane = blockIdx.y;
     const float* src = q + (plane * n + x) * (size_t)n * n;
@@ -390,8 +392,9 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd, int
 template <int MAXV, int G>
 __global__ void __launch_bounds__(256)
 k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
-        FftDesc fd, int n, int nhz, int C, int T, unsigned grp_mask) {
+        FftDesc fdg, int n, int nhz, int C, int T, unsigned grp_mask) {
     extern __shared__ float2 lds[];
+    const FftDesc fd = stage_twiddles(fdg, lds + n * T);
     const int plane = n * nhz;
     const int j0 = blockIdx.x * T;
     const int nt = min(T, plane - j0);
@@ -464,10 +467,11 @@ k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2*
 template <int MAXE, int G>
 __global__ void __launch_bounds__(256)
 k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
-           FftDesc fd, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
+           FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
     extern __shared__ float2 lds[];
     float2* A = lds;
     float2* Bf = lds + n * T;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
     const int id = blockIdx.x;
     const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
     const int b = rem >> 3;
@@ -483,35 +487,53 @@ k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     for (int g = 0; g < G; ++g)
 #pragma unroll
         for (int m = 0; m < MAXE; ++m) acc[g][m] = make_float2(0.f, 0.f);
-    for (int c = 0; c < C; ++c) {
+    // software pipeline: the query tile of channel c + 1 and the target tile
+    // of channel c are in flight while the FFT of channel c runs in LDS
+    // (the workgroup barrier waits only for LDS traffic)
+    float2 v[MAXE], tv[MAXE];
+    // element e = threadIdx.x + 256 m of the tile: global offset, or -1 past
+    // the tile; bit 30 marks a slab outside the query support
+    int off[MAXE];
+#pragma unroll
+    for (int m = 0; m < MAXE; ++m) {
+        const int e = threadIdx.x + m * 256;
+        off[m] = -1;
+        if (e < ne) {
+            const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+            off[m] = xx * plane + t;
+            if (xx < lo || xx >= lo + w) off[m] |= (1 << 30);
+        }
+    }
+    auto load_query = [&](int c) {
         const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane + j0;
-        {
-            float2 v[MAXE];
 #pragma unroll
-            for (int m = 0; m < MAXE; ++m) {
-                const int e = threadIdx.x + m * 256;
-                if (e < ne) {
-                    const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
-                    // slabs outside [lo, lo + w) of the query spectra are zero
-                    v[m] = (xx >= lo && xx < lo + w) ? src[(size_t)xx * plane + t] : make_float2(0.f, 0.f);
-                }
-            }
+        for (int m = 0; m < MAXE; ++m) {
+            if (off[m] >= 0)
+                v[m] = (off[m] & (1 << 30)) ? make_float2(0.f, 0.f) : src[off[m]];
+        }
+    };
+    load_query(0);
+    for (int c = 0; c < C; ++c) {
 #pragma unroll
-            for (int m = 0; m < MAXE; ++m) {
-                const int e = threadIdx.x + m * 256;
-                if (e < ne) A[e] = v[m];
-            }
+        for (int m = 0; m < MAXE; ++m) {
+            const int e = threadIdx.x + m * 256;
+            if (e < ne) A[e] = v[m];
         }
         __syncthreads();
+        {
+            const float2* tc = tgt + (size_t)c * n * plane + j0;
+#pragma unroll
+            for (int m = 0; m < MAXE; ++m)
+                if (off[m] >= 0) tv[m] = tc[off[m] & ~(1 << 30)];
+        }
+        if (c + 1 < C) load_query(c + 1);
         const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
-        const float2* tc = tgt + (size_t)c * n * plane + j0;
         const int g = (grp_mask >> c) & 1;
 #pragma unroll
         for (int m = 0; m < MAXE; ++m) {
             const int e = threadIdx.x + m * 256;
             if (e < ne) {
-                const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
-                const float2 pr = cmul(r[e], tc[(size_t)xx * plane + t]);
+                const float2 pr = cmul(r[e], tv[m]);
 #pragma unroll
                 for (int gg = 0; gg < G; ++gg)
                     if (gg == g) acc[gg][m] = cadd(acc[gg][m], pr);
@@ -532,10 +554,82 @@ k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
 #pragma unroll
         for (int m = 0; m < MAXE; ++m) {
             const int e = threadIdx.x + m * 256;
-            if (e < ne) {
-                const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
-                dst[(size_t)xx * plane + t] = cconj(r[e]);
+            if (off[m] >= 0) dst[off[m] & ~(1 << 30)] = cconj(r[e]);
+        }
+        __syncthreads();
+    }
+}
+
+// x pass, LDS-accumulating variant: same work as k_xcorr_pp, but the group
+// sums live in LDS (one more tile buffer) instead of registers, so a thread
+// holds a single butterfly; smaller tiles give several workgroups per CU.
+template <int G>
+__global__ void __launch_bounds__(256)
+k_xcorr_lds(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+            FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
+    extern __shared__ float2 lds[];
+    const int id = blockIdx.x;
+    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
+    const int b = rem >> 3;
+    const int tile = grp * 8 + (rem & 7);
+    if (tile >= ntiles) return;
+    float2* A = lds;
+    float2* Bf = lds + n * T;
+    float2* acc = lds + 2 * n * T;                 // [G][n * T]
+    const FftDesc fd = stage_twiddles(fdg, lds + (2 + G) * n * T);
+    const int plane = n * nhz;
+    const int j0 = tile * T;
+    const int nt = min(T, plane - j0);
+    const int ne = n * nt;
+    const float ntinv = 1.0f / (float)nt;
+    for (int e = threadIdx.x; e < G * ne; e += 256) acc[e] = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; ++c) {
+        const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane + j0;
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+            float2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * 256;
+                v[u] = make_float2(0.f, 0.f);
+                if (e < ne) {
+                    const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+                    if (xx >= lo && xx < lo + w) v[u] = src[(size_t)xx * plane + t];
+                }
             }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e0 + u * 256 < ne) A[e0 + u * 256] = v[u];
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+        const float2* tc = tgt + (size_t)c * n * plane + j0;
+        float2* ag = acc + ((grp_mask >> c) & 1) * ne;
+        for (int e0 = threadIdx.x; e0 < ne; e0 += 4 * 256) {
+            float2 tv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * 256;
+                if (e < ne) {
+                    const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+                    tv[u] = tc[(size_t)xx * plane + t];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * 256;
+                if (e < ne) ag[e] = cadd(ag[e], cmul(r[e], tv[u]));
+            }
+        }
+        __syncthreads();
+    }
+    for (int g = 0; g < G; ++g) {
+        for (int e = threadIdx.x; e < ne; e += 256) A[e] = cconj(acc[g * ne + e]);
+        __syncthreads();
+        const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+        float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
+        for (int e = threadIdx.x; e < ne; e += 256) {
+            const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+            dst[(size_t)xx * plane + t] = cconj(r[e]);
         }
         __syncthreads();
     }
@@ -545,9 +639,10 @@ k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
 // Optionally writes the real correlation (prob-mode mixing).
 template <int MAXV, int NT>
 __global__ void __launch_bounds__(NT)
-k_inv_yz(const float2* __restrict__ sum, FftDesc fd, int n, int nhz, int G,
+k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
          unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
     extern __shared__ float2 lds[];
+    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
     __shared__ float s_v[16];
     __shared__ uint32_t s_i[16];
     const int x = blockIdx.x;
@@ -710,7 +805,7 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     const int nhz = n / 2 + 1;
     int nthr, maxv;
     slab_shape(n, nhz, &nthr, &maxv);
-    const size_t lds = (size_t)n * nhz * sizeof(float2);
+    const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
@@ -726,7 +821,7 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     const int nhz = n / 2 + 1;
     int nthr, maxv;
     slab_shape(n, nhz, &nthr, &maxv);
-    const size_t lds = (size_t)n * nhz * sizeof(float2);
+    const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -745,35 +840,54 @@ static int xcorr_T(int n) {
     return T;
 }
 
+constexpr int kXcorrMaxE = 12;   // tile values per thread in the x pass
+
 static int xcorr_pp_T(int n) {
-    // n * T <= 256 threads * 16 values; T <= 16 keeps two tiles of LDS small
-    int T = 4096 / n;
+    // n * T <= 256 threads * kXcorrMaxE values; T <= 16 keeps two tiles of LDS small
+    int T = 256 * kXcorrMaxE / n;
     if (T > 16) T = 16;
     if (T < 1) T = 1;
     return T;
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
-                 int nb, int C, int G, unsigned grp_mask, int lo, int w, hipStream_t st) {
+                 int nb, int C, int G, unsigned grp_mask, int lo, int w, hipStream_t st,
+                 int xvariant = 0, int xT = 0) {
     const int nhz = n / 2 + 1;
     if (tgt == nullptr) {
         // target preparation: forward x FFT + conj per channel (once per search)
         const int T = xcorr_T(n);
         VX_REQUIRE(n * T <= 256 * 16, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
-        const size_t lds = (size_t)n * T * sizeof(float2);
+        const size_t lds = ((size_t)n * T + fd.ntw) * sizeof(float2);
         dim3 grid((n * nhz + T - 1) / T, nb);
         hipLaunchKernelGGL((k_xcorr<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, C, T, grp_mask);
         VX_LAUNCH_CHECK();
         return VX_OK;
     }
+    if (xvariant == 1) {
+        int T = xT > 0 ? xT : 8;
+        if ((size_t)(2 + G) * n * T * sizeof(float2) + fd.ntw * sizeof(float2) > kMaxLds) T = 1;
+        const size_t lds = ((size_t)(2 + G) * n * T + fd.ntw) * sizeof(float2);
+        VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
+        const int ntiles = (n * nhz + T - 1) / T;
+        dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
+        if (lds > 65536) {
+            if (G == 1) VX_HIP(hipFuncSetAttribute((const void*)k_xcorr_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            else        VX_HIP(hipFuncSetAttribute((const void*)k_xcorr_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        }
+        if (G == 1) hipLaunchKernelGGL((k_xcorr_lds<1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+        else        hipLaunchKernelGGL((k_xcorr_lds<2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+        VX_LAUNCH_CHECK();
+        return VX_OK;
+    }
     const int T = xcorr_pp_T(n);
-    VX_REQUIRE(n * T <= 256 * 16, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
-    const size_t lds = 2 * (size_t)n * T * sizeof(float2);
+    VX_REQUIRE(n * T <= 256 * kXcorrMaxE, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
+    const size_t lds = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
     const int ntiles = (n * nhz + T - 1) / T;
     const int groups8 = (ntiles + 7) / 8;
     dim3 grid((unsigned)(groups8 * 8 * nb));
-    if (G == 1) hipLaunchKernelGGL((k_xcorr_pp<16, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-    else        hipLaunchKernelGGL((k_xcorr_pp<16, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    if (G == 1) hipLaunchKernelGGL((k_xcorr_pp<kXcorrMaxE, 1>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    else        hipLaunchKernelGGL((k_xcorr_pp<kXcorrMaxE, 2>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -824,6 +938,8 @@ struct vx_plan {
     bool have_target, have_search, mix;
     bool q_dirty;     // q / qd hold non-zero values outside the support box
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
+    int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating
+    int xT;           // x-pass tile width (columns), 0 = default
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
 };
@@ -1001,6 +1117,21 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     return VX_OK;
 }
 
+extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
+    VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
+    switch (key) {
+        case VX_TUNE_XPASS_VARIANT:
+            VX_REQUIRE(value == 0 || value == 1, VX_ERR_ARG, "x-pass variant must be 0 or 1");
+            p->xvariant = value; break;
+        case VX_TUNE_XPASS_TILE:
+            VX_REQUIRE(value >= 0 && value <= 64, VX_ERR_ARG, "x-pass tile must be in [0, 64]");
+            p->xT = value; break;
+        default:
+            VX_REQUIRE(false, VX_ERR_ARG, "unknown tuning key %d", key);
+    }
+    return VX_OK;
+}
+
 extern "C" int vx_set_prob_stats(vx_plan* p, int enable, double alpha, double vstd, double vave,
                                  double pstd, double pave) {
     VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
@@ -1045,7 +1176,8 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
         rc = launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         VX_MARK(2);
-        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st);
+        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
+                          p->xvariant, p->xT);
         if (rc != VX_OK) return rc;
         VX_MARK(3);
         rc = launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk,

@@ -138,6 +138,13 @@ class SearchPlan:
                     channels=int(out[4]), workspace_bytes=int(out[5]), box_lo=int(out[6]),
                     box_width=int(out[7]))
 
+    def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None):
+        """Select kernel schedules (results are identical for every choice)."""
+        if xpass_variant is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_VARIANT, int(xpass_variant))
+        if xpass_tile is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_TILE, int(xpass_tile))
+
     def set_target(self, data, vec, prob=None, ave=0.0):
         nat.call("vx_set_target", self._h, _p(data), _p(vec), _p(prob), float(ave), _stream())
 
