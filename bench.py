@@ -107,6 +107,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-rows", type=int, default=512)
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--rows", type=int, default=0, help="score only the first ROWS rows per step (profiling)")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="first run a device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -119,9 +122,17 @@ def main():
     from vesper_python_amd.geometry import calc_angle_comb, rotation_rows
 
     engine.require_gpu()
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb * 262144, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     w, n1, n2 = prepare(args.workload)
     n = int(n1.xdim)
     angles = calc_angle_comb(w["A"])
+    if args.rows:
+        angles = angles[:args.rows]
     R = len(angles)
     dev = torch.device("cuda", 0)
     tdata, tvec = n1._dev

@@ -93,10 +93,12 @@ int vx_plan_destroy(vx_plan* plan);
  * Kernel schedule of a plan (the autotuner's knobs; results do not depend on
  * them).  VX_TUNE_XPASS_VARIANT: 0 = x pass accumulating channels in
  * registers, 1 = in LDS.  VX_TUNE_XPASS_TILE: columns per x-pass tile
- * (0 = default).
+ * (0 = default).  Grids whose (y, z) slab exceeds LDS always use the pencil
+ * passes.
  */
 #define VX_TUNE_XPASS_VARIANT 1
 #define VX_TUNE_XPASS_TILE 2
+#define VX_TUNE_FORCE_PENCIL 3 /* 1: separate z and y passes even where a slab fits LDS */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

@@ -248,3 +248,57 @@ def test_support_box_pruning_is_exact(gpu, n):
     o32, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref32")
     o64, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref64")
     _check_rows(res[0][0][:, 0], res[0][1][:, 0], o32, o64, f"prune n={n}")
+
+
+@pytest.mark.parametrize("n", [24, 27, 40])
+def test_pencil_passes_match_slab(gpu, n):
+    """The pencil passes (separate z and y transforms, used where a slab
+    exceeds LDS) give the slab path's results, and both match the oracle."""
+    rng = np.random.default_rng(700 + n)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.3)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    outside = np.linalg.norm(g, axis=-1) > n / 3
+    sd[outside] = 0
+    sv[outside] = 0
+    angles = np.concatenate([[[0, 0, 0]], rng.uniform(0, 360, (6, 3))])
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    res = []
+    for pencil in (False, True):
+        _say("pencil", n, pencil)
+        plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=3)
+        plan.tune(pencil=pencil)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+    assert np.allclose(res[0][0], res[1][0], rtol=2e-6)
+    o32, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref64")
+    for b, t in res:
+        _check_rows(b[:, 0], t[:, 0], o32, o64, f"pencil n={n}")
+
+
+def test_large_grid_uses_pencils(gpu):
+    """N = 250 (slab of 250 x 126 complex exceeds LDS): scores and shift of a
+    translated self-copy through the pencil path."""
+    n = 250
+    rng = np.random.default_rng(11)
+    td = np.zeros((n, n, n))
+    tv = np.zeros((n, n, n, 3), np.float32)
+    blob = rng.uniform(0.2, 1.0, (40, 40, 40)) * (rng.uniform(size=(40, 40, 40)) < 0.3)
+    td[100:140, 100:140, 100:140] = blob
+    v = rng.standard_normal((40, 40, 40, 3))
+    v /= np.linalg.norm(v, axis=-1, keepdims=True)
+    tv[100:140, 100:140, 100:140] = (v * (blob != 0)[..., None]).astype(np.float32)
+    shift = (7, 240, 19)
+    sd = np.roll(td, shift, axis=(0, 1, 2))
+    sv = np.roll(tv, shift, axis=(0, 1, 2))
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=1)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+    best, trans = plan.score_rows(torch.from_numpy(rotation_rows([[0, 0, 0]])).cuda())
+    assert tuple(trans[0, 0].cpu().tolist()) == shift
+    exp = np.sum(tv.astype(np.float64) ** 2)
+    assert abs(best[0, 0].item() / n ** 3 - exp) < 1e-4 * exp

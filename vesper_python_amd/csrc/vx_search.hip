@@ -728,6 +728,195 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pencil passes for grids whose (y, z) slab does not fit in LDS
+// ---------------------------------------------------------------------------
+
+// Forward r2c along z for a chunk of P line pairs of one x-slab: two real
+// rows packed into one complex line, FFT, unpacked into half spectra.
+// Only slabs x in [lo, lo + w) and pairs covering rows [lo, lo + w).
+__global__ void __launch_bounds__(256)
+k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w, int P) {
+    extern __shared__ float2 lds[];
+    float2* A = lds;
+    float2* Bf = lds + P * n;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * P * n);
+    const int x = lo + blockIdx.y;
+    const size_t plane = blockIdx.z;
+    const int l0 = (lo >> 1) + blockIdx.x * P;
+    const int l1 = min(l0 + P, ((lo + w - 1) >> 1) + 1);
+    const int np = l1 - l0;
+    if (np <= 0) return;
+    const float* src = q + (plane * n + x) * (size_t)n * n;
+    const float ninv = 1.0f / (float)n;
+    for (int e = threadIdx.x; e < np * n; e += 256) {
+        const int ll = fdiv(e, n, ninv), z = e - ll * n, l = l0 + ll;
+        const float re = src[(size_t)(2 * l) * n + z];
+        const float im = (2 * l + 1 < n) ? src[(size_t)(2 * l + 1) * n + z] : 0.f;
+        A[e] = make_float2(re, im);
+    }
+    __syncthreads();
+    const float2* r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
+    float2* out = spec + (plane * n + x) * (size_t)n * nhz;
+    const float hinv = 1.0f / (float)nhz;
+    for (int u = threadIdx.x; u < np * nhz; u += 256) {
+        const int ll = fdiv(u, nhz, hinv), k = u - ll * nhz, l = l0 + ll;
+        const float2 zk = r[ll * n + k];
+        const float2 wv = r[ll * n + (k == 0 ? 0 : n - k)];
+        out[(size_t)(2 * l) * nhz + k] = make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
+        if (2 * l + 1 < n) out[(size_t)(2 * l + 1) * nhz + k] = make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x));
+    }
+}
+
+// In-place complex FFT along y of a tile of T kz-columns of one x-slab.
+// Forward: rows outside [ylo, ylo + yw) are read as zero.  Inverse
+// (unnormalised): conj on load and store.
+__global__ void __launch_bounds__(256)
+k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n, int nhz, int T, int inverse, int ylo, int yw, int xlo) {
+    extern __shared__ float2 lds[];
+    float2* A = lds;
+    float2* Bf = lds + n * T;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
+    const int k0 = blockIdx.x * T;
+    const int nt = min(T, nhz - k0);
+    const int x = xlo + blockIdx.y;
+    float2* base = buf + ((size_t)blockIdx.z * n + x) * (size_t)n * nhz + k0;
+    const float ntinv = 1.0f / (float)nt;
+    for (int e = threadIdx.x; e < n * nt; e += 256) {
+        const int y = fdiv(e, nt, ntinv), t = e - y * nt;
+        float2 v = make_float2(0.f, 0.f);
+        if (y >= ylo && y < ylo + yw) v = base[(size_t)y * nhz + t];
+        A[e] = inverse ? cconj(v) : v;
+    }
+    __syncthreads();
+    const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+    for (int e = threadIdx.x; e < n * nt; e += 256) {
+        const int y = fdiv(e, nt, ntinv), t = e - y * nt;
+        base[(size_t)y * nhz + t] = inverse ? cconj(r[e]) : r[e];
+    }
+}
+
+// Inverse c2r along z for a chunk of P line pairs (rows 2l, 2l+1 of the
+// y-inverted half spectrum), then max + first argmax -> atomicMax.
+__global__ void __launch_bounds__(256)
+k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G, int P,
+        unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
+    extern __shared__ float2 lds[];
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    float2* A = lds;
+    float2* Bf = lds + P * n;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * P * n);
+    const int x = blockIdx.y;
+    const int bg = blockIdx.z;
+    const int npair = (n + 1) >> 1;
+    const int l0 = blockIdx.x * P;
+    const int np = min(P, npair - l0);
+    const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
+    const int half = n >> 1;
+    const bool even = (n & 1) == 0;
+    const float ninv = 1.0f / (float)n;
+    for (int u = threadIdx.x; u < np * n; u += 256) {
+        const int ll = fdiv(u, n, ninv), k = u - ll * n, l = l0 + ll;
+        const bool hasb = (2 * l + 1) < n;
+        const int kk = (k <= half) ? k : n - k;
+        float2 a = src[(size_t)(2 * l) * nhz + kk];
+        float2 bb = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+        float2 zc;
+        if (k <= half) {
+            if (k == 0 || (even && k == half)) { a.y = 0.f; bb.y = 0.f; }
+            zc = make_float2(a.x - bb.y, -a.y - bb.x);       // conj(A + iB)
+        } else {
+            zc = make_float2(a.x + bb.y, a.y - bb.x);        // conj(conj A' + i conj B')
+        }
+        A[u] = zc;
+    }
+    __syncthreads();
+    const float2* r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    const size_t n3 = (size_t)n * n * n;
+    for (int e = threadIdx.x; e < np * n; e += 256) {
+        const int ll = fdiv(e, n, ninv), z = e - ll * n, l = l0 + ll;
+        const float2 f = r[e];
+        const uint32_t ia = ((uint32_t)x * n + 2 * l) * n + z;
+        if (better(f.x, ia, bv, bi)) { bv = f.x; bi = ia; }
+        if (real_out) real_out[(size_t)bg * n3 + ia] = f.x;
+        if (2 * l + 1 < n) {
+            const uint32_t ib = ia + n;
+            if (better(-f.y, ib, bv, bi)) { bv = -f.y; bi = ib; }
+            if (real_out) real_out[(size_t)bg * n3 + ib] = -f.y;
+        }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) {
+        const int b = bg / G, g = bg - b * G;
+        atomicMax(&keys[(size_t)b * nkeys + g], argmax_key(bv, bi));
+    }
+}
+
+static int pencil_pairs(int n) {
+    int P = 4096 / n;          // ~4k complex values per workgroup
+    if (P < 1) P = 1;
+    if (P > 32) P = 32;
+    return P;
+}
+
+static int pencil_T(int n) {
+    int T = 4096 / n;
+    if (T < 1) T = 1;
+    if (T > 16) T = 16;
+    return T;
+}
+
+template <typename K>
+static int allow_lds_any(K kernel, size_t bytes) {
+    if (bytes > 65536) VX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+    return VX_OK;
+}
+
+int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                      hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    const int P = pencil_pairs(n);
+    const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
+    VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
+    const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
+    allow_lds_any(k_fwd_z, lz);
+    hipLaunchKernelGGL(k_fwd_z, dim3((l1 - l0 + P - 1) / P, w, nplanes), dim3(256), lz, st, q, spec, fd, n, nhz,
+                       lo, w, P);
+    VX_LAUNCH_CHECK();
+    const int T = pencil_T(n);
+    const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
+    VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
+    allow_lds_any(k_axis_y, ly);
+    hipLaunchKernelGGL(k_axis_y, dim3((nhz + T - 1) / T, w, nplanes), dim3(256), ly, st, spec, fd, n, nhz, T, 0,
+                       2 * l0, 2 * (l1 - l0), lo);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys, int nkeys,
+                      float* real_out, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    const int T = pencil_T(n);
+    const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
+    VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
+    allow_lds_any(k_axis_y, ly);
+    hipLaunchKernelGGL(k_axis_y, dim3((nhz + T - 1) / T, n, nbg), dim3(256), ly, st, sum, fd, n, nhz, T, 1,
+                       0, n, 0);
+    VX_LAUNCH_CHECK();
+    const int P = pencil_pairs(n);
+    const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
+    VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
+    allow_lds_any(k_inv_z, lz);
+    const int npair = (n + 1) / 2;
+    hipLaunchKernelGGL(k_inv_z, dim3((npair + P - 1) / P, n, nbg), dim3(256), lz, st, sum, fd, n, nhz, G, P, keys,
+                       nkeys, real_out);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
 // Prob-mode mixed score (find_best_trans_mixed, VESPER_1_prob_new.py:1530-1545)
 // in float32 with float32-rounded constants, no FMA:
 //   mixed = fl(fl(a * fl(fl(v - vave) / vstd)) + fl(b * fl(fl(p - pave) / pstd)))
@@ -866,7 +1055,7 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
     }
     if (xvariant == 1) {
         int T = xT > 0 ? xT : 8;
-        if ((size_t)(2 + G) * n * T * sizeof(float2) + fd.ntw * sizeof(float2) > kMaxLds) T = 1;
+        while (T > 1 && ((size_t)(2 + G) * n * T + fd.ntw) * sizeof(float2) > kMaxLds) T >>= 1;
         const size_t lds = ((size_t)(2 + G) * n * T + fd.ntw) * sizeof(float2);
         VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
         const int ntiles = (n * nhz + T - 1) / T;
@@ -940,6 +1129,7 @@ struct vx_plan {
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
     int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating
     int xT;           // x-pass tile width (columns), 0 = default
+    int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
 };
@@ -972,13 +1162,12 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     VX_REQUIRE(mode >= VX_MODE_VEC && mode <= VX_MODE_PROB, VX_ERR_ARG, "bad mode %d", mode);
     VX_REQUIRE(batch >= 1 && batch <= 4096, VX_ERR_ARG, "bad batch %d", batch);
     const int nhz = n / 2 + 1;
-    VX_REQUIRE((size_t)n * nhz * sizeof(float2) <= kMaxLds, VX_ERR_UNSUPPORTED,
-               "grid edge %d: slab of %zu bytes exceeds LDS", n, (size_t)n * nhz * sizeof(float2));
     vx_plan* p = new (std::nothrow) vx_plan();
     VX_REQUIRE(p != nullptr, VX_ERR_NOMEM, "host allocation failed");
     memset(p, 0, sizeof(*p));
     p->n = n; p->nhz = nhz; p->mode = mode; p->batch = batch; p->flags = flags;
     p->lo = 0; p->w = n; p->q_dirty = true;
+    p->xvariant = 1; p->xT = 0;   // LDS-accumulating x pass, default tile
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
@@ -987,6 +1176,11 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     }
     int rc = fft_desc_create(n, &p->fd, &p->twid);
     if (rc != VX_OK) { plan_free(p); return rc; }
+    {
+        int nthr, maxv;
+        slab_shape(n, nhz, &nthr, &maxv);
+        p->pencil = (maxv == 0 || ((size_t)n * nhz + p->fd.ntw) * sizeof(float2) > kMaxLds) ? 1 : 0;
+    }
     const size_t n3 = (size_t)n * n * n;
     const size_t nh = (size_t)n * n * nhz;
     const int B = batch;
@@ -1107,7 +1301,8 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     VX_LAUNCH_CHECK();
     // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
     // spec holds B*C planes >= C planes.
-    int rc = launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st);
+    int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st)
+                       : launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st);
     if (rc != VX_OK) return rc;
     // treat the C channels as "batch" 1 with C channels, forward-conj mode
     rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, 0, n, st);
@@ -1126,6 +1321,16 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
         case VX_TUNE_XPASS_TILE:
             VX_REQUIRE(value >= 0 && value <= 64, VX_ERR_ARG, "x-pass tile must be in [0, 64]");
             p->xT = value; break;
+        case VX_TUNE_FORCE_PENCIL:
+            if (value) p->pencil = 1;
+            else {
+                int nthr, maxv;
+                slab_shape(p->n, p->nhz, &nthr, &maxv);
+                VX_REQUIRE(maxv > 0 && ((size_t)p->n * p->nhz + p->fd.ntw) * sizeof(float2) <= kMaxLds,
+                           VX_ERR_UNSUPPORTED, "grid edge %d needs the pencil passes", p->n);
+                p->pencil = 0;
+            }
+            break;
         default:
             VX_REQUIRE(false, VX_ERR_ARG, "unknown tuning key %d", key);
     }
@@ -1173,15 +1378,17 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
             VX_LAUNCH_CHECK();
         }
         VX_MARK(1);
-        rc = launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
+        rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st)
+                       : launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         VX_MARK(2);
         rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
                           p->xvariant, p->xT);
         if (rc != VX_OK) return rc;
         VX_MARK(3);
-        rc = launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk,
-                           (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr, st);
+        float* ro = (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr;
+        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st)
+                       : launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st);
         if (rc != VX_OK) return rc;
         VX_MARK(4);
         if (p->mode == VX_MODE_PROB && p->mix) {

@@ -138,8 +138,11 @@ class SearchPlan:
                     channels=int(out[4]), workspace_bytes=int(out[5]), box_lo=int(out[6]),
                     box_width=int(out[7]))
 
-    def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None):
+    def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None,
+             pencil: bool | None = None):
         """Select kernel schedules (results are identical for every choice)."""
+        if pencil is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_FORCE_PENCIL, int(bool(pencil)))
         if xpass_variant is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_VARIANT, int(xpass_variant))
         if xpass_tile is not None:
