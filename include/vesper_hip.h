@@ -99,6 +99,11 @@ int vx_plan_destroy(vx_plan* plan);
 #define VX_TUNE_XPASS_VARIANT 1
 #define VX_TUNE_XPASS_TILE 2
 #define VX_TUNE_FORCE_PENCIL 3 /* 1: separate z and y passes even where a slab fits LDS */
+#define VX_TUNE_PIN_BOX_WIDTH 4 /* > 0: fix the support box to [PIN_BOX_LO, PIN_BOX_LO + width) so
+                                   launch shapes stay constant across queries (captured graphs);
+                                   vx_set_search fails if a query's support does not fit it;
+                                   0 = per query.  Set VX_TUNE_PIN_BOX_LO first. */
+#define VX_TUNE_PIN_BOX_LO 5
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

@@ -1,0 +1,80 @@
+"""Captured search steps (HIP graphs) and the multi-pair driver: a replayed
+step returns the eager step's results bit for bit, replay after replay with
+eager work in between, and pairs of uneven grid size searched through
+per-edge graphs match the per-pair eager search and the oracle."""
+import contextlib
+import io
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import vesper_ref as ref
+from vesper_python_amd import _native as nat
+from vesper_python_amd import batch, engine, synth
+from vesper_python_amd import VESPER_1_prob_new as V
+from vesper_python_amd import main as cli
+from vesper_python_amd.geometry import rotation_rows
+
+pytestmark = pytest.mark.gpu
+
+
+def _maps(rng, n, density=0.3):
+    data = rng.uniform(0.1, 1.0, (n, n, n)) * (rng.uniform(size=(n, n, n)) < density)
+    vec = rng.standard_normal((n, n, n, 3))
+    vec /= np.linalg.norm(vec, axis=-1, keepdims=True)
+    return data, (vec * (data != 0)[..., None]).astype(np.float32)
+
+
+def test_graph_replay_matches_eager(gpu):
+    n = 24
+    rng = np.random.default_rng(21)
+    td, tv = _maps(rng, n)
+    sd, sv = _maps(rng, n)
+    angles = rng.uniform(0, 360, (40, 3))
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=8)
+    plan.set_target(engine.to_dev(td, torch.float64), engine.to_dev(tv, torch.float32))
+    plan.set_search(engine.to_dev(sd, torch.float64), engine.to_dev(sv, torch.float32))
+    eb, et = plan.score_rows(rot)
+    eb, et = eb.cpu().numpy(), et.cpu().numpy()
+    erows = engine.topn(torch.from_numpy(eb[:, 0].copy()).cuda(), 10)
+    gs = batch.GraphedSearch(plan, rot, 10)
+    for rep in range(4):
+        print("[case] replay", rep, flush=True)
+        b, t, rows = gs.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(b.cpu().numpy(), eb) and np.array_equal(t.cpu().numpy(), et)
+        assert np.array_equal(rows.cpu().numpy(), erows)
+        plan.score_rows(rot[:8])                # eager work between replays
+        torch.cuda.synchronize()
+
+
+def _resampled_pair(tmp, seed, a_edge, natoms, s):
+    p = synth.make_pair(seed, a_edge=a_edge, natoms=natoms, b_edge=40, nsub=80)
+    paths = synth.write_pair(p, str(tmp / f"p{seed}"))
+    with contextlib.redirect_stdout(io.StringIO()):
+        m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], p.a.threshold, p.b.threshold, s, 8.0)
+        n1 = V.fastVEC(m1, n1, 8.0)
+        n2 = V.fastVEC(m2, n2, 8.0)
+    return n1, n2
+
+
+def test_uneven_pairs_graph_equals_eager_and_oracle(gpu, tmp_path):
+    pairs = [_resampled_pair(tmp_path, 40 + i, e, a, 4.0)
+             for i, (e, a) in enumerate([(60, 200), (72, 260), (60, 220), (84, 300)])]
+    edges = [p[0].xdim for p in pairs]
+    assert len(set(edges)) >= 2, edges
+    g = batch.search_pairs(pairs, ang=30.0, topn=10, use_graph=True)
+    e = batch.search_pairs(pairs, ang=30.0, topn=10, use_graph=False)
+    for a, b in zip(g, e):
+        assert np.array_equal(a["scores"], b["scores"]) and np.array_equal(a["trans"], b["trans"])
+        assert a["top"].tolist() == b["top"].tolist()
+    # one pair against the oracle's exhaustive search (no refinement)
+    t, s = pairs[1]
+    grid = lambda m: ref.MapGrid(data=np.asarray(m.data, np.float64), xwidth=np.float32(m.xwidth), orig=m.orig,
+                                 vec=m.vec, cent=m.cent, xdim=m.xdim, ave=m.ave, std=m.std,
+                                 std_norm_ave=m.std_norm_ave)
+    o = ref.search_map_fft(grid(t), grid(s), TopN=10, ang=30.0, prec="ref32", refine=False)
+    assert g[1]["top"].tolist() == list(o["topn"])
+    assert np.allclose(g[1]["scores"], o["scores"], rtol=1e-4, atol=1e-6 * np.abs(o["scores"]).max())

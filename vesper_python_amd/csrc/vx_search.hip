@@ -1130,6 +1130,8 @@ struct vx_plan {
     int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating
     int xT;           // x-pass tile width (columns), 0 = default
     int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
+    int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
+    int pin_lo;
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
 };
@@ -1275,6 +1277,13 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
         int lo = (int)floor(c - r), hi = (int)ceil(c + r);
         if (lo < 0) lo = 0;
         if (hi > p->n - 1) hi = p->n - 1;
+        if (p->pin_w > 0) {
+            const int plo = p->pin_lo, phi = plo + p->pin_w - 1;
+            VX_REQUIRE(lo >= plo && hi <= phi, VX_ERR_STATE,
+                       "query support [%d, %d] exceeds the pinned box [%d, %d]", lo, hi, plo, phi);
+            lo = plo;
+            hi = phi;
+        }
         p->lo = lo;
         p->w = hi - lo + 1;
     }
@@ -1307,8 +1316,12 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     // treat the C channels as "batch" 1 with C channels, forward-conj mode
     rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, 0, n, st);
     if (rc != VX_OK) return rc;
+    // the target channels used the query planes: clear them for the gather,
+    // which writes only inside the support box
+    VX_HIP(hipMemsetAsync(p->q, 0, (size_t)p->batch * p->C * n3 * sizeof(float), st));
+    if (p->qd) VX_HIP(hipMemsetAsync(p->qd, 0, (size_t)p->batch * n3 * sizeof(double), st));
     p->have_target = true;
-    p->q_dirty = true;
+    p->q_dirty = false;
     return VX_OK;
 }
 
@@ -1321,6 +1334,15 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
         case VX_TUNE_XPASS_TILE:
             VX_REQUIRE(value >= 0 && value <= 64, VX_ERR_ARG, "x-pass tile must be in [0, 64]");
             p->xT = value; break;
+        case VX_TUNE_PIN_BOX_LO:
+            VX_REQUIRE(value >= 0 && value < p->n, VX_ERR_ARG, "pinned box start must be in [0, n)");
+            p->pin_lo = value;
+            break;
+        case VX_TUNE_PIN_BOX_WIDTH:
+            VX_REQUIRE(value >= 0 && p->pin_lo + value <= p->n, VX_ERR_ARG, "pinned box exceeds the grid");
+            p->pin_w = value;
+            if (value > 0) { p->lo = p->pin_lo; p->w = value; }
+            break;
         case VX_TUNE_FORCE_PENCIL:
             if (value) p->pencil = 1;
             else {

@@ -148,6 +148,12 @@ class SearchPlan:
         if xpass_tile is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_TILE, int(xpass_tile))
 
+    def pin_box(self, lo: int, width: int):
+        """Fix the support box to [lo, lo + width) on every axis (width 0 =
+        recompute per query)."""
+        nat.call("vx_plan_tune", self._h, nat.VX_TUNE_PIN_BOX_LO, int(lo))
+        nat.call("vx_plan_tune", self._h, nat.VX_TUNE_PIN_BOX_WIDTH, int(width))
+
     def set_target(self, data, vec, prob=None, ave=0.0):
         nat.call("vx_set_target", self._h, _p(data), _p(vec), _p(prob), float(ave), _stream())
 
