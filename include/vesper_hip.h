@@ -196,6 +196,56 @@ int vx_threshold_dmax(float* data_dev, int sx, int sy, int sz, float th, double*
 /* laplacian_filter (VESPER_1_prob_new.py:494-526) on an [n][n][n] float64 grid. */
 int vx_laplacian(int n, const double* in_dev, double* out_dev, vx_stream stream);
 
+
+/* ------------------------------------------------------------------------
+ * Learned rotation proposal.  No counterpart in the reference (which scores
+ * every Euler row, VESPER_1_prob_new.py:644-655): a 4-layer MLP predicts a
+ * row's DOT score so that only the best fraction of rows is FFT-scored.
+ * bf16 matrices are raw uint16 storage; fp32 accumulation on MFMA.
+ * ------------------------------------------------------------------------ */
+
+/* C = op(A) op(B): A [M x K] (transA: stored K x M), B [K x N] (transB:
+ * stored N x K), row-major bf16.  K a multiple of 16.
+ * epilogue 0: C fp32 = AB;  1: C bf16 = relu(AB + bias);  2: C bf16 =
+ * AB where aux (bf16 [M x N], ld ldaux) > 0 else 0;  3: C fp32 += AB (atomic). */
+int vx_gemm_bf16(int M, int N, int K, const void* A_dev, int lda, int transA, const void* B_dev, int ldb,
+                 int transB, void* C_dev, int ldc, int epilogue, const float* bias_dev, const void* aux_dev,
+                 int ldaux, vx_stream stream);
+
+/* output layer pred[m] = sum_k X[m][k] w[k] + b[0] (X bf16 [M x K], ld ldx) */
+int vx_mlp_head_fwd(int M, int K, const void* X_dev, int ldx, const float* w_dev, const float* b_dev,
+                    float* pred_dev, vx_stream stream);
+
+/* its backward with mean-squared loss: dw += .., db += .., dX = d w (X > 0),
+ * loss += mean (pred - y)^2 */
+int vx_mlp_head_bwd(int M, int K, const void* X_dev, int ldx, const float* w_dev, const float* pred_dev,
+                    const float* y_dev, float* dw_dev, float* db_dev, void* dX_dev, float* loss_dev,
+                    vx_stream stream);
+
+/* out[n] += sum_m G[m][n] (bf16 G) */
+int vx_colsum_bf16(int M, int N, const void* G_dev, int ldg, float* out_dev, vx_stream stream);
+
+/* Adam step (bias-corrected, step >= 1) on fp32 p; refreshes the bf16 copy if non-NULL */
+int vx_adam(int n, float* p_dev, const float* g_dev, float* m1_dev, float* m2_dev, void* p_bf16_dev, float lr,
+            float beta1, float beta2, float eps, int step, float weight_decay, vx_stream stream);
+
+int vx_cast_bf16(int n, const float* in_dev, void* out_bf16_dev, vx_stream stream);
+
+/* 32 rotational moments of a resampled map (data float64 [n^3], vec float32
+ * [n^3 x 3]) about the rotation centre n/2: centroid, second moments, mean
+ * vector, position-vector and vector-vector moments, 5 invariants. */
+int vx_mlp_descriptors(int n, const double* data_dev, const float* vec_dev, float* out32_dev, vx_stream stream);
+
+/* Feature rows (bf16 [M x ld], ld >= 68): quaternion, target moments, query
+ * moments rotated by R = F^T of each row of rot_dev [M x 9]. */
+int vx_mlp_features(int M, int ld, const float* rot_dev, const float* quat_dev, const float* tdesc_dev,
+                    const float* qdesc_dev, void* X_dev, vx_stream stream);
+
+/* Rows whose prediction ranks in about the top k (4096-bin histogram
+ * threshold; ties at the threshold bin are all kept).  rows_dev [M];
+ * *nsel_host = number written (>= k unless M < k).  Synchronises. */
+int vx_select_top(int M, const float* pred_dev, int k, int* rows_dev, int* nsel_host, vx_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

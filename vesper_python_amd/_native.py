@@ -49,6 +49,16 @@ SIGNATURES = {
     "vx_get_score": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P]),
     "vx_laplacian": (_I, [_I, _P, _P, _P]),
     "vx_threshold_dmax": (_I, [_P, _I, _I, _I, ctypes.c_float, _P, _P]),
+    "vx_gemm_bf16": (_I, [_I, _I, _I, _P, _I, _I, _P, _I, _I, _P, _I, _I, _P, _P, _I, _P]),
+    "vx_mlp_head_fwd": (_I, [_I, _I, _P, _I, _P, _P, _P, _P]),
+    "vx_mlp_head_bwd": (_I, [_I, _I, _P, _I, _P, _P, _P, _P, _P, _P, _P, _P]),
+    "vx_colsum_bf16": (_I, [_I, _I, _P, _I, _P, _P]),
+    "vx_adam": (_I, [_I, _P, _P, _P, _P, _P, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                     _I, ctypes.c_float, _P]),
+    "vx_cast_bf16": (_I, [_I, _P, _P, _P]),
+    "vx_mlp_descriptors": (_I, [_I, _P, _P, _P, _P]),
+    "vx_mlp_features": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
+    "vx_select_top": (_I, [_I, _P, _I, _P, _P, _P]),
 }
 
 

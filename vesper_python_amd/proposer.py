@@ -1,0 +1,227 @@
+"""Learned rotation proposal: which Euler rows deserve a full FFT score.
+
+A 4-layer MLP (80 -> 256 -> 256 -> 256 -> 1; bf16 weights and activations,
+fp32 accumulation on MFMA through vx_gemm_bf16) maps a rotation's quaternion
+plus 64 rotational moments (32 of the target field, 32 of the query field
+rotated by that rotation) to a predicted DOT score.  It is trained with the
+library's own forward/backward kernels and Adam on score landscapes the
+exhaustive search produces: across training pairs, and per query pair on a
+coarse landscape (every row of a coarser angle step) before proposing rows of
+the fine step.  Only the predicted top fraction of fine rows is FFT-scored;
+the exhaustive search stays available as the fallback and as the judge
+(top-10 recall).  No reference counterpart: the reference scores every row
+(VESPER_1_prob_new.py:644-655).
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import torch
+from scipy.spatial.transform import Rotation
+
+from . import _native as nat
+from . import engine
+from .geometry import calc_angle_comb, rotation_rows
+
+IN = 80            # 4 + 32 + 32 features, padded to a multiple of 16
+H = 256
+
+
+def _p(t):
+    return engine._p(t)
+
+
+def _st():
+    return engine._stream()
+
+
+def pad64(m: int) -> int:
+    return (m + 63) // 64 * 64
+
+
+def quaternions(angles) -> np.ndarray:
+    """Unit quaternion (x, y, z, w), w >= 0, of the content rotation
+    R_xyz(angle) that a row applies to the query (euler2rot,
+    VESPER_1_prob_new.py:1549-1551)."""
+    q = Rotation.from_euler("xyz", np.atleast_2d(angles), degrees=True).as_quat()
+    q[q[:, 3] < 0] *= -1
+    return np.ascontiguousarray(q, dtype=np.float32)
+
+
+def descriptors(data: torch.Tensor, vec: torch.Tensor) -> torch.Tensor:
+    """32 rotational moments of a resampled map (device float32 [32])."""
+    out = torch.empty(32, dtype=torch.float32, device=data.device)
+    nat.call("vx_mlp_descriptors", int(data.shape[0]), _p(data), _p(vec), _p(out), _st())
+    return out
+
+
+def features(rot: torch.Tensor, quat: torch.Tensor, tdesc: torch.Tensor, qdesc: torch.Tensor) -> torch.Tensor:
+    """bf16 feature rows [pad64(R), IN] (zero rows past R)."""
+    R = int(rot.shape[0])
+    X = torch.zeros((pad64(R), IN), dtype=torch.bfloat16, device=rot.device)
+    nat.call("vx_mlp_features", R, IN, _p(rot), _p(quat), _p(tdesc), _p(qdesc), _p(X), _st())
+    return X
+
+
+class RotationMLP:
+    """The proposal network; parameters are fp32 masters with bf16 copies."""
+
+    def __init__(self, seed: int = 0, device=None):
+        dev = device or engine.require_gpu()
+        rng = np.random.default_rng(seed)
+        fan = [IN, H, H]
+        self.W = [torch.from_numpy((rng.standard_normal((k, H)) * math.sqrt(2.0 / k)).astype(np.float32)).to(dev)
+                  for k in fan]
+        self.b = [torch.zeros(H, dtype=torch.float32, device=dev) for _ in fan]
+        self.w4 = torch.from_numpy((rng.standard_normal(H) * math.sqrt(1.0 / H)).astype(np.float32)).to(dev)
+        self.b4 = torch.zeros(1, dtype=torch.float32, device=dev)
+        self.Wb = [torch.empty((k, H), dtype=torch.bfloat16, device=dev) for k in fan]
+        for w, wb in zip(self.W, self.Wb):
+            nat.call("vx_cast_bf16", w.numel(), _p(w), _p(wb), _st())
+        self.params = self.W + self.b + [self.w4, self.b4]
+        self.m1 = [torch.zeros_like(p) for p in self.params]
+        self.m2 = [torch.zeros_like(p) for p in self.params]
+        self.steps = 0
+        self.dev = dev
+        self._bufs = {}
+
+    # -- buffers ---------------------------------------------------------------
+    def _get(self, Mp: int):
+        if Mp not in self._bufs:
+            z = lambda: torch.zeros((Mp, H), dtype=torch.bfloat16, device=self.dev)
+            self._bufs[Mp] = dict(A=[z(), z(), z()], D=[z(), z(), z()],
+                                  pred=torch.zeros(Mp, dtype=torch.float32, device=self.dev))
+        return self._bufs[Mp]
+
+    # -- forward ---------------------------------------------------------------
+    def forward(self, X: torch.Tensor, M: int) -> torch.Tensor:
+        """Predicted scores of the first M rows of X (bf16 [Mp, IN])."""
+        Mp = int(X.shape[0])
+        buf = self._get(Mp)
+        inp, ldi = X, IN
+        for l in range(3):
+            K = IN if l == 0 else H
+            nat.call("vx_gemm_bf16", Mp, H, K, _p(inp), ldi, 0, _p(self.Wb[l]), H, 0, _p(buf["A"][l]), H, 1,
+                     _p(self.b[l]), None, 0, _st())
+            inp, ldi = buf["A"][l], H
+        nat.call("vx_mlp_head_fwd", M, H, _p(buf["A"][2]), H, _p(self.w4), _p(self.b4), _p(buf["pred"]), _st())
+        return buf["pred"][:M]
+
+    # -- training --------------------------------------------------------------
+    def grads(self, X: torch.Tensor, y: torch.Tensor, M: int):
+        """Mean-squared-loss gradients of every parameter (fp32, same order as
+        self.params) and the loss, by the library's backward kernels."""
+        Mp = int(X.shape[0])
+        buf = self._get(Mp)
+        pred = self.forward(X, M)
+        g = [torch.zeros_like(p) for p in self.params]
+        gW, gb, gw4, gb4 = g[0:3], g[3:6], g[6], g[7]
+        loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        A, D = buf["A"], buf["D"]
+        D[2][M:].zero_()      # rows past M take no part (the head writes rows < M)
+        # output layer; D[2] = dL/dZ3 (through the ReLU of A3)
+        nat.call("vx_mlp_head_bwd", M, H, _p(A[2]), H, _p(self.w4), _p(pred), _p(y), _p(gw4), _p(gb4),
+                 _p(D[2]), _p(loss), _st())
+        for l in (2, 1, 0):
+            inp, K = (X, IN) if l == 0 else (A[l - 1], H)
+            # dW_l += inp^T dZ_l
+            nat.call("vx_gemm_bf16", K, H, Mp, _p(inp), K, 1, _p(D[l]), H, 0, _p(gW[l]), H, 3, None, None, 0, _st())
+            nat.call("vx_colsum_bf16", Mp, H, _p(D[l]), H, _p(gb[l]), _st())
+            if l > 0:
+                # dZ_{l-1} = (dZ_l W_l^T) * (A_{l-1} > 0)
+                nat.call("vx_gemm_bf16", Mp, H, H, _p(D[l]), H, 0, _p(self.Wb[l]), H, 1, _p(D[l - 1]), H, 2,
+                         None, _p(A[l - 1]), H, _st())
+        return g, loss
+
+    def train_step(self, X, y, M, lr=1e-3, wd=0.0):
+        g, loss = self.grads(X, y, M)
+        self.steps += 1
+        bf = self.Wb + [None] * 5
+        for p, gp, a, b, pb in zip(self.params, g, self.m1, self.m2, bf):
+            nat.call("vx_adam", p.numel(), _p(p), _p(gp), _p(a), _p(b), _p(pb), lr, 0.9, 0.999, 1e-8,
+                     self.steps, wd, _st())
+        return loss
+
+    def fit(self, X, y, M, epochs=20, batch=2048, lr=1e-3, seed=0):
+        """Minibatch Adam over the rows of X (bf16 [Mp, IN]) / y (fp32 [M])."""
+        rng = np.random.default_rng(seed)
+        bp = pad64(min(batch, M))
+        xb = torch.zeros((bp, IN), dtype=torch.bfloat16, device=self.dev)
+        yb = torch.zeros(bp, dtype=torch.float32, device=self.dev)
+        losses = []
+        for _ in range(epochs):
+            perm = torch.from_numpy(rng.permutation(M)).to(self.dev)
+            tot = torch.zeros(1, device=self.dev)
+            for s in range(0, M, batch):
+                idx = perm[s:s + batch]
+                m = int(idx.numel())
+                xb[:m] = X.index_select(0, idx)
+                xb[m:] = 0
+                yb[:m] = y.index_select(0, idx)
+                tot += self.train_step(xb, yb, m, lr) * m
+            losses.append(float(tot.item()) / M)
+        return losses
+
+    def state(self) -> dict:
+        return {f"p{i}": p.cpu().numpy() for i, p in enumerate(self.params)}
+
+    def load(self, st: dict):
+        for i, p in enumerate(self.params):
+            p.copy_(torch.from_numpy(st[f"p{i}"]))
+        for w, wb in zip(self.W, self.Wb):
+            nat.call("vx_cast_bf16", w.numel(), _p(w), _p(wb), _st())
+
+
+def landscape(plan: engine.SearchPlan, rot: torch.Tensor) -> torch.Tensor:
+    """Exhaustive float32 best scores of every row (the training labels)."""
+    best, _ = plan.score_rows(rot)
+    return best[:, 0].contiguous()
+
+
+def zscore(s: torch.Tensor) -> torch.Tensor:
+    s = s.double()
+    return ((s - s.mean()) / (s.std() + 1e-30)).float()
+
+
+class Proposer:
+    """Proposal search for one pair held in a search plan."""
+
+    def __init__(self, plan: engine.SearchPlan, tdata, tvec, sdata, svec, mlp: RotationMLP):
+        self.plan = plan
+        self.mlp = mlp
+        self.td = descriptors(tdata, tvec)
+        self.qd = descriptors(sdata, svec)
+
+    def rows_features(self, angles):
+        rot = torch.from_numpy(rotation_rows(angles)).to(self.td.device)
+        quat = torch.from_numpy(quaternions(angles)).to(self.td.device)
+        return rot, features(rot, quat, self.td, self.qd)
+
+    def fine_tune(self, coarse_angles, epochs=30, lr=1e-3):
+        """Train on this pair's own coarse landscape (every row of a coarse
+        angle step, FFT-scored by the plan)."""
+        rot, X = self.rows_features(coarse_angles)
+        y = zscore(landscape(self.plan, rot))
+        return self.mlp.fit(X, y, int(rot.shape[0]), epochs=epochs, batch=512, lr=lr)
+
+    def propose(self, angles, frac=0.05, topn=10):
+        """FFT-score only the predicted top `frac` of `angles`; returns
+        (top rows into angles, their float32 best, number of rows scored)."""
+        rot, X = self.rows_features(angles)
+        R = int(rot.shape[0])
+        pred = self.mlp.forward(X, R)
+        k = max(topn, int(math.ceil(frac * R)))
+        rows = torch.empty(R, dtype=torch.int32, device=rot.device)
+        nsel = np.zeros(1, dtype=np.int32)
+        nat.call("vx_select_top", R, _p(pred), k, _p(rows), nsel.ctypes.data_as(__import__("ctypes").c_void_p), _st())
+        sel = rows[:int(nsel[0])].long().sort().values
+        best, _ = self.plan.score_rows(rot.index_select(0, sel).contiguous())
+        b = best[:, 0].contiguous()
+        t = engine.topn(b, topn)
+        return sel[torch.from_numpy(t.astype(np.int64)).to(sel.device)].cpu().numpy(), b, int(sel.numel())
+
+
+def recall(a, b) -> float:
+    a, b = list(a), list(b)
+    return len(set(a) & set(b)) / max(1, len(b))
