@@ -96,6 +96,31 @@ def cpu_baseline(n1, n2, angles, budget_s: float, nrows_max: int):
                        f"oracle ref32 rot_mrc + scipy.fft float32 r2c/c2r, 1 thread, {dt:.1f} s")
 
 
+def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
+    """Learned proposal on the same pair: train on the -A 30 landscape, then
+    FFT-score 5 % of the -A `ang` rows in two rounds with retraining; recall
+    of the exhaustive top-10 (`best` holds the exhaustive scores) and
+    rotation-equivalents per second (R rows covered / wall time of the whole
+    proposal pipeline, training included)."""
+    import torch
+    from vesper_python_amd import engine, proposer
+    from vesper_python_amd.geometry import calc_angle_comb
+    truth = engine.topn(best[:, 0].contiguous(), 10)
+    fine = calc_angle_comb(ang)
+    coarse = calc_angle_comb(30.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    pr = proposer.Proposer(plan, *n1._dev, *n2._dev, proposer.RotationMLP(seed=7))
+    top, scored, ncoarse = proposer.propose_rounds(pr, coarse, fine, frac=0.05, rounds=2)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    rate = R / dt
+    return {"recall_top10": proposer.recall(top, truth), "fine_rows": R, "fft_rows_scored": scored + ncoarse,
+            "seconds": round(dt, 3), "rotation_equivalents_per_s": round(rate, 1),
+            "speedup_vs_plain": round(rate / plain_rate, 2),
+            "method": "MLP (80-256-256-256-1, bf16/MFMA) trained on the -A 30 landscape, 2 rounds x 2.5 % of rows"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +135,7 @@ def main():
     ap.add_argument("--rows", type=int, default=0, help="score only the first ROWS rows per step (profiling)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a device copy of this many MiB (counter calibration)")
+    ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -190,6 +216,9 @@ def main():
                 "support_box": [info["box_lo"], info["box_width"]],
                 "stage_GBps": {k: round(sb[k] / (stage_ms[k] / launches / 1e3) / 1e9, 1) for k in sb}}
 
+    proposal = None
+    if not args.no_proposal and not args.rows:
+        proposal = measure_proposal(plan, n1, n2, w["A"], best, R, value)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(n1, n2, angles, args.cpu_seconds, 64)
@@ -200,7 +229,7 @@ def main():
            "config": {"workload": f"{args.workload}: {w['text']}", "grid_edge": n, "rows": R,
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
-           "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu}
+           "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(rec), flush=True)

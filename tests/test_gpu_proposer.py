@@ -134,6 +134,11 @@ def test_proposal_recall_beats_random(gpu, tmp_path):
         pr.fine_tune(coarse, epochs=40, lr=2e-3)
         top, _, scored = pr.propose(fine, frac=0.05, topn=10)
         rec_mlp.append(proposer.recall(top, truth))
+        # two rounds of 2.5 % with retraining on the rows scored in between
+        top2, scored2, _ = proposer.propose_rounds(proposer.Proposer(plan, *n1._dev, *n2._dev,
+                                                                     proposer.RotationMLP(seed=seed)),
+                                                   coarse, fine, frac=0.05, rounds=2)
+        print("[recall2]", seed, proposer.recall(top2, truth), "scored", scored2, flush=True)
         rand = rng.choice(len(fine), scored, replace=False)
         b = best[:, 0].cpu().numpy()[rand]
         rtop = rand[np.argsort(-b, kind="stable")[:10]]

@@ -222,6 +222,51 @@ class Proposer:
         return sel[torch.from_numpy(t.astype(np.int64)).to(sel.device)].cpu().numpy(), b, int(sel.numel())
 
 
+def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, topn=10, epochs=40, lr=2e-3,
+                   seed=0):
+    """Coarse landscape -> train -> propose a share of the rows -> FFT-score
+    them -> retrain on everything scored so far -> propose the next share of
+    the rows not yet scored ... (`rounds` shares of frac / rounds each).
+    Returns (top rows into angles, number of fine rows FFT-scored, coarse rows)."""
+    dev = pr.td.device
+    crot, cX = pr.rows_features(coarse_angles)
+    cy = zscore(landscape(pr.plan, crot))
+    rot, X = pr.rows_features(angles)
+    R = int(rot.shape[0])
+    scored = torch.zeros(R, dtype=torch.bool, device=dev)
+    best_all = torch.full((R,), -float("inf"), dtype=torch.float32, device=dev)
+    trainX, trainy = [cX[:int(crot.shape[0])]], [cy]
+    share = max(topn, int(math.ceil(frac * R / rounds)))
+    for r in range(rounds):
+        Xt = torch.cat(trainX)
+        yt = torch.cat(trainy)
+        pr.mlp.fit(Xt, yt, int(yt.numel()), epochs=epochs if r == 0 else max(5, epochs // 4), batch=512, lr=lr,
+                   seed=seed + r)
+        pred = pr.mlp.forward(X, R).clone()
+        if bool(scored.any()):
+            # rows already scored drop just below the lowest unscored prediction
+            # (finite, so the selection histogram keeps its resolution)
+            low = pred[~scored].min()
+            pred[scored] = low - 1e-3 * low.abs() - 1e-3
+        rows = torch.empty(R, dtype=torch.int32, device=dev)
+        nsel = np.zeros(1, dtype=np.int32)
+        nat.call("vx_select_top", R, _p(pred), share, _p(rows), nsel.ctypes.data_as(__import__("ctypes").c_void_p),
+                 _st())
+        sel = rows[:int(nsel[0])].long()
+        sel = sel[~scored[sel]].sort().values
+        if sel.numel() == 0:
+            break
+        b, _ = pr.plan.score_rows(rot.index_select(0, sel).contiguous())
+        best_all[sel] = b[:, 0]
+        scored[sel] = True
+        trainX.append(X.index_select(0, sel))
+        # labels on the coarse landscape's scale
+        cs = landscape(pr.plan, crot).double()
+        trainy.append(((b[:, 0].double() - cs.mean()) / (cs.std() + 1e-30)).float())
+    top = engine.topn(best_all, topn)
+    return top, int(scored.sum().item()), int(crot.shape[0])
+
+
 def recall(a, b) -> float:
     a, b = list(a), list(b)
     return len(set(a) & set(b)) / max(1, len(b))
