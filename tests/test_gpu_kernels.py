@@ -302,3 +302,75 @@ def test_large_grid_uses_pencils(gpu):
     assert tuple(trans[0, 0].cpu().tolist()) == shift
     exp = np.sum(tv.astype(np.float64) ** 2)
     assert abs(best[0, 0].item() / n ** 3 - exp) < 1e-4 * exp
+
+
+# ------------------------------------------------ split-bf16 MFMA x pass
+
+@pytest.mark.parametrize("n,flags", [(16, 0), (24, nat.VX_FLAG_NO_PRUNE), (35, 0), (40, 0)])
+def test_mfma_xpass_matches_oracle(gpu, n, flags):
+    """x-axis DFTs on MFMA with split bf16 operands and fp32 accumulation:
+    best scores within 1e-4 of the FP64 oracle; a translation may differ only
+    at an FP64 near-tie."""
+    rng = np.random.default_rng(900 + n)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.5)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    outside = np.linalg.norm(g, axis=-1) > n / 3
+    sd[outside] = 0
+    sv[outside] = 0
+    angles = np.concatenate([[[0, 0, 0]], rng.uniform(0, 360, (7, 3))])
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    _say("mfma xpass", n, flags)
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=3, flags=flags)
+    plan.tune(xpass_variant=2)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+    b, t = plan.score_rows(rot)
+    o32, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, "VecProduct", "ref64")
+    _check_rows(b[:, 0].cpu().numpy(), t[:, 0].cpu().numpy(), o32, o64, f"mfma n={n}")
+
+
+def test_mfma_xpass_cc_mode(gpu):
+    n = 22
+    rng = np.random.default_rng(950)
+    td, tv = _maps(rng, n, 0.4)
+    sd, sv = _maps(rng, n, 0.4)
+    angles = rng.uniform(0, 360, (6, 3))
+    o32, t = _oracle_rows(td, tv, sd, sv, angles, "CC", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles, "CC", "ref64")
+    _say("mfma xpass cc", n)
+    plan = engine.SearchPlan(n, nat.VX_MODE_CC, batch=4)
+    plan.tune(xpass_variant=2)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), None, t.ave)
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), None, t.ave)
+    b, tr = plan.score_rows(torch.from_numpy(rotation_rows(angles)).cuda())
+    _check_rows(b[:, 0].cpu().numpy(), tr[:, 0].cpu().numpy(), o32, o64, "mfma cc")
+
+
+def test_mfma_xpass_agrees_with_fft_at_n96(gpu):
+    """Grid edge 96 with a pruned support box: the MFMA x pass and the FFT x
+    pass give best scores within 1e-4 relative of each other on every row."""
+    n = 96
+    rng = np.random.default_rng(960)
+    td, tv = _maps(rng, n, 0.05)
+    sd, sv = _maps(rng, n, 0.2)
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - n / 2
+    outside = np.linalg.norm(g, axis=-1) > n / 4
+    sd[outside] = 0
+    sv[outside] = 0
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (24, 3)))).cuda()
+    res = []
+    for variant in (1, 2):
+        _say("mfma vs fft n=96 variant", variant)
+        plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=8)
+        plan.tune(xpass_variant=variant)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+        b, t = plan.score_rows(rot)
+        res.append((b[:, 0].cpu().numpy().astype(np.float64), t[:, 0].cpu().numpy()))
+    (bf, tf), (bm, tm) = res
+    rel = np.abs(bm - bf) / np.abs(bf).max()
+    print("[mfma] max rel diff", rel.max(), "trans agree", np.mean(np.all(tf == tm, axis=1)), flush=True)
+    assert rel.max() <= RTOL

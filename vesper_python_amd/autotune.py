@@ -4,8 +4,10 @@ For a plan (grid edge N, mode, batch), times every candidate schedule of the
 x pass (register- or LDS-accumulating kernel, tile width) on a sample of the
 actual rotation rows with the library's per-stage HIP-event profiler
 (vx_score_rows_profile), keeps the fastest, and caches it per
-(device, N, mode, batch) in a JSON file.  Every schedule computes identical
-results; only the time differs.
+(device, N, mode, batch) in a JSON file.  The FFT schedules (variants 0, 1)
+compute identical results; the split-bf16 MFMA DFT x pass (variant 2) is
+accepted only if its best scores on the sample agree with the FFT schedule's
+within 1e-4 of the largest score.
 """
 from __future__ import annotations
 
@@ -14,6 +16,7 @@ import os
 import sys
 import threading
 
+import numpy as np
 import torch
 
 CACHE = os.environ.get("VX_TUNE_CACHE",
@@ -21,10 +24,15 @@ CACHE = os.environ.get("VX_TUNE_CACHE",
 _lock = threading.Lock()
 
 
-def candidates(n: int):
+APPROX_RTOL = 1e-4
+
+
+def candidates(n: int, groups: int = 1):
     out = [dict(xpass_variant=0, xpass_tile=0)]
     for t in (2, 4, 8, 12, 16):
         out.append(dict(xpass_variant=1, xpass_tile=t))
+    if groups == 1 and n <= 192:
+        out.append(dict(xpass_variant=2, xpass_tile=0))
     return out
 
 
@@ -65,14 +73,22 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
     nrows = min(int(rot.shape[0]), max(plan.batch, sample_rows // plan.batch * plan.batch))
     sample = rot[:nrows]
     results = []
-    for cand in candidates(plan.n):
+    exact = None
+    for cand in candidates(plan.n, plan.groups):
         try:
             plan.tune(**cand)
-            plan.score_rows(sample[:plan.batch])          # warm-up of this schedule
+            got, _ = plan.score_rows(sample[:plan.batch])  # warm-up of this schedule
             ms = plan.profile_rows(sample)
         except Exception as e:  # a schedule the device rejects is skipped
             if verbose:
                 print("[tune] skip", cand, e, file=sys.stderr)
+            continue
+        got = got.double().cpu().numpy()
+        if cand["xpass_variant"] != 2:
+            exact = got if exact is None else exact
+        elif exact is None or np.max(np.abs(got - exact)) > APPROX_RTOL * max(np.max(np.abs(exact)), 1e-30):
+            if verbose:
+                print("[tune] reject", cand, "(accuracy)", file=sys.stderr)
             continue
         total = sum(ms.values())
         results.append((total, cand, ms))

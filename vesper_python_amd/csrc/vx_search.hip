@@ -635,6 +635,273 @@ k_xcorr_lds(const float2* __restrict__ spec, const float2* __restrict__ tgt, flo
     }
 }
 
+// ---------------------------------------------------------------------------
+// x pass on MFMA: the x-axis DFTs as real-form complex matrix products with
+// split bf16 operands (v = hi + lo, hi = bf16(v), lo = bf16(v - hi)); each
+// product is hi*hi + hi*lo + lo*hi accumulated in fp32 (the dropped lo*lo
+// term is ~2^-16 of the product).  Per (tile of 64 columns, rotation):
+//   Y_c = F X_c          F [2NP x 2KP]: rows (Re, Im) of kx, columns
+//                         (Re, Im) of the query slabs x in [lo, lo + KP)
+//   P   = sum_c Y_c * tgt_c             (fp32 VALU; tgt holds conj spectra)
+//   Z   = G P            G [2NP x 2NP]: unnormalised inverse DFT
+// Wave v owns the row tiles i = v, v + 4, ... (Re rows, and their Im rows
+// i + NPT) for all 64 columns, so no wave repeats another's A operand.  The
+// matrices are stored pre-swizzled in MFMA fragment order (one 512-byte run
+// per 16x16 tile, read straight from L2); the data operand is staged in LDS
+// as [column][k] split bf16.  v_mfma_f32_16x16x16_bf16 lane layout: lane l
+// holds A[l % 16][4 (l / 16) + v], B[4 (l / 16) + v][l % 16] and
+// D[4 (l / 16) + v][l % 16].
+// ---------------------------------------------------------------------------
+
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float floatx4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ void split_bf16(float v, unsigned short& hi, unsigned short& lo) {
+    hi = bf16_rne(v);
+    lo = bf16_rne(v - __uint_as_float((uint32_t)hi << 16));
+}
+
+constexpr int kMfmaCols = 64;      // columns of the y-z plane per workgroup
+constexpr int kMfmaPad = 8;        // LDS row padding (bf16 elements)
+constexpr int kMfmaMaxNPT = 12;    // row tiles of 16: grid edges up to 192
+
+// acc[p][h][j] += A(row tile i + h NPT, k step ks) * B(column tile j), split bf16
+template <int MAXP, int NPT>
+__device__ __forceinline__ void mfma_split_step(floatx4_t (&acc)[MAXP][2][4], const unsigned short* __restrict__ Ah,
+                                                const unsigned short* __restrict__ Al, int KS, int ks,
+                                                const unsigned short* bhp, const unsigned short* blp, int stride,
+                                                int wave, int lane) {
+    const int l16 = lane & 15, l4 = (lane >> 4) * 4;
+    bf16x4_t bh[4], bl[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        bh[j] = *(const bf16x4_t*)&bhp[(16 * j + l16) * stride + 16 * ks + l4];
+        bl[j] = *(const bf16x4_t*)&blp[(16 * j + l16) * stride + 16 * ks + l4];
+    }
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+        const int i = wave + 4 * p;
+        if (i >= NPT) break;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const size_t ro = ((size_t)((i + h * NPT) * KS + ks) * 64 + lane) * 4;
+            const bf16x4_t ah = *(const bf16x4_t*)&Ah[ro];
+            const bf16x4_t al = *(const bf16x4_t*)&Al[ro];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[j], acc[p][h][j], 0, 0, 0);
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[j], acc[p][h][j], 0, 0, 0);
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[j], acc[p][h][j], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <int NPT>
+__global__ void __launch_bounds__(256)
+k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+             const unsigned short* __restrict__ Fh, const unsigned short* __restrict__ Fl,
+             const unsigned short* __restrict__ Gh, const unsigned short* __restrict__ Gl,
+             int n, int nhz, int KP, int lo, int w, int C, int nb, int ntiles) {
+    constexpr int NP = 16 * NPT;
+    constexpr int MAXP = (NPT + 3) / 4;
+    constexpr int NP2 = 2 * NP;
+    extern __shared__ unsigned short sm[];
+    const int id = blockIdx.x;
+    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
+    const int b = rem >> 3;
+    const int tile = grp * 8 + (rem & 7);
+    if (tile >= ntiles) return;
+    const int plane = n * nhz;
+    const int j0 = tile * kMfmaCols;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int l16 = lane & 15, l4 = (lane >> 4) * 4;
+    const int K2 = 2 * KP, KS = K2 / 16;
+    const int SK = K2 + kMfmaPad;
+    unsigned short* xh = sm;
+    unsigned short* xl = sm + kMfmaCols * SK;
+    float P[MAXP][4][4][2];
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) { P[p][j][v][0] = 0.f; P[p][j][v][1] = 0.f; }
+    for (int c = 0; c < C; ++c) {
+        // stage X_c^T: column t, k < KP -> Re of slab lo + k, k >= KP -> Im
+        const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane + j0;
+        for (int e = threadIdx.x; e < kMfmaCols * KP; e += 256) {
+            const int k = e >> 6, t = e & 63;
+            float2 v = make_float2(0.f, 0.f);
+            if (k < w && j0 + t < plane) v = src[(size_t)(lo + k) * plane + t];
+            unsigned short h, l;
+            split_bf16(v.x, h, l);
+            xh[t * SK + k] = h; xl[t * SK + k] = l;
+            split_bf16(v.y, h, l);
+            xh[t * SK + KP + k] = h; xl[t * SK + KP + k] = l;
+        }
+        __syncthreads();
+        floatx4_t acc[MAXP][2][4];
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p)
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < KS; ++ks)
+            mfma_split_step<MAXP, NPT>(acc, Fh, Fl, KS, ks, xh, xl, SK, wave, lane);
+        // P += Y_c * tgt_c
+        const float2* tc = tgt + (size_t)c * n * plane + j0;
+#pragma unroll
+        for (int p = 0; p < MAXP; ++p) {
+            const int i = wave + 4 * p;
+            if (i >= NPT) break;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int kx = 16 * i + l4 + v, t = 16 * j + l16;
+                    if (kx < n && j0 + t < plane) {
+                        const float2 tv = tc[(size_t)kx * plane + t];
+                        const float yr = acc[p][0][j][v], yi = acc[p][1][j][v];
+                        P[p][j][v][0] += yr * tv.x - yi * tv.y;
+                        P[p][j][v][1] += yr * tv.y + yi * tv.x;
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    // stage P^T as the inverse's data operand: column t, r < NP -> Re kx = r, else Im
+    const int SP = NP2 + kMfmaPad;
+    unsigned short* ph = sm;
+    unsigned short* pl = sm + kMfmaCols * SP;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+        const int i = wave + 4 * p;
+        if (i >= NPT) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int kx = 16 * i + l4 + v, t = 16 * j + l16;
+                unsigned short h, l;
+                split_bf16(P[p][j][v][0], h, l);
+                ph[t * SP + kx] = h; pl[t * SP + kx] = l;
+                split_bf16(P[p][j][v][1], h, l);
+                ph[t * SP + NP + kx] = h; pl[t * SP + NP + kx] = l;
+            }
+    }
+    __syncthreads();
+    floatx4_t acc[MAXP][2][4];
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < NP2 / 16; ++ks)
+        mfma_split_step<MAXP, NPT>(acc, Gh, Gl, NP2 / 16, ks, ph, pl, SP, wave, lane);
+    float2* dst = out + (size_t)b * n * (size_t)plane + j0;
+#pragma unroll
+    for (int p = 0; p < MAXP; ++p) {
+        const int i = wave + 4 * p;
+        if (i >= NPT) break;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int x = 16 * i + l4 + v, t = 16 * j + l16;
+                if (x < n && j0 + t < plane)
+                    dst[(size_t)x * plane + t] = make_float2(acc[p][0][j][v], acc[p][1][j][v]);
+            }
+    }
+}
+
+// Fragment-ordered split bf16 DFT matrices of the MFMA x pass (host side).
+// rows 2 NP, columns 2 KP; element (r, k) of tile (r / 16, k / 16) at
+// ((tile_r * KS + tile_k) * 64 + lane) * 4 + v, lane = r % 16 + 16 ((k % 16) / 4),
+// v = k % 4.  fwd: F (columns are query slabs x = lo + k); else G.
+static void build_dft_matrix(int n, int NP, int KP, int lo, int w, bool fwd,
+                             std::vector<unsigned short>& hi, std::vector<unsigned short>& lov) {
+    const int R2 = 2 * NP, K2 = fwd ? 2 * KP : 2 * NP, KS = K2 / 16, half = K2 / 2;
+    hi.assign((size_t)R2 * K2, 0);
+    lov.assign((size_t)R2 * K2, 0);
+    auto to_bf16 = [](float f) {
+        uint32_t u;
+        memcpy(&u, &f, 4);
+        u += 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(u >> 16);
+    };
+    for (int r = 0; r < R2; ++r) {
+        const int kx = r % NP;
+        const bool im_row = r >= NP;
+        if (kx >= n) continue;
+        for (int k = 0; k < K2; ++k) {
+            const int m = k % half;
+            const bool im_col = k >= half;
+            const int x = fwd ? lo + m : m;
+            if ((fwd && m >= w) || x >= n) continue;
+            const double ang = 2.0 * M_PI * (double)(((long long)kx * x) % n) / n;
+            const double cs = cos(ang), sn = sin(ang);
+            // fwd: Y = sum X e^{-i ang}; inverse: Z = sum P e^{+i ang}
+            double val;
+            if (!im_row) val = im_col ? (fwd ? sn : -sn) : cs;
+            else         val = im_col ? cs : (fwd ? -sn : sn);
+            const float f = (float)val;
+            const unsigned short h = to_bf16(f);
+            uint32_t hu = (uint32_t)h << 16;
+            float hf;
+            memcpy(&hf, &hu, 4);
+            const unsigned short l = to_bf16(f - hf);
+            const int lane = (r % 16) + 16 * ((k % 16) / 4);
+            const size_t idx = ((size_t)((r / 16) * KS + k / 16) * 64 + lane) * 4 + (k % 4);
+            hi[idx] = h;
+            lov[idx] = l;
+        }
+    }
+}
+
+static size_t xcorr_mfma_lds(int NP, int KP) {
+    const int s = std::max(2 * KP, 2 * NP) + kMfmaPad;
+    return (size_t)kMfmaCols * s * 2 * sizeof(unsigned short);
+}
+
+template <int NPT>
+static void launch_xcorr_mfma_t(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
+                                float2* out, const unsigned short* const* mats, int n, int nhz, int KP,
+                                int lo, int w, int C, int nb, int ntiles) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_xcorr_mfma<NPT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_xcorr_mfma<NPT>), grid, dim3(256), lds, st, spec, tgt, out, mats[0], mats[1], mats[2],
+                       mats[3], n, nhz, KP, lo, w, C, nb, ntiles);
+}
+
+int launch_xcorr_mfma(const float2* spec, const float2* tgt, float2* out, const unsigned short* const* mats,
+                      int n, int nb, int C, int KP, int lo, int w, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    const int NPT = (n + 15) / 16;
+    VX_REQUIRE(NPT <= kMfmaMaxNPT, VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", n);
+    VX_REQUIRE(KP % 8 == 0 && KP >= w && KP <= 16 * NPT, VX_ERR_STATE, "bad MFMA x-pass width");
+    const size_t lds = xcorr_mfma_lds(16 * NPT, KP);
+    VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", n);
+    const int ntiles = (n * nhz + kMfmaCols - 1) / kMfmaCols;
+    dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
+    switch (NPT) {
+#define VX_NPT(k) case k: launch_xcorr_mfma_t<k>(grid, lds, st, spec, tgt, out, mats, n, nhz, KP, lo, w, C, nb, ntiles); break;
+        VX_NPT(1) VX_NPT(2) VX_NPT(3) VX_NPT(4) VX_NPT(5) VX_NPT(6)
+        VX_NPT(7) VX_NPT(8) VX_NPT(9) VX_NPT(10) VX_NPT(11) VX_NPT(12)
+#undef VX_NPT
+        default: break;
+    }
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
 // Inverse: sum[bg][x] ([ky][kz]) -> real slab ([y][z]) -> argmax key.
 // Optionally writes the real correlation (prob-mode mixing).
 template <int MAXV, int NT>
@@ -1127,11 +1394,14 @@ struct vx_plan {
     bool have_target, have_search, mix;
     bool q_dirty;     // q / qd hold non-zero values outside the support box
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
-    int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating
+    int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating, 2 MFMA DFT
     int xT;           // x-pass tile width (columns), 0 = default
     int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
     int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
     int pin_lo;
+    unsigned short* dft;     // MFMA x pass: split bf16 F (hi, lo) then G (hi, lo), fragment order
+    size_t dft_elems[2];     // elements of one F matrix, one G matrix
+    int dft_lo, dft_w, dft_kp; // box the F matrices were built for (dft_w = 0: none)
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
 };
@@ -1139,7 +1409,7 @@ struct vx_plan {
 static void plan_free(vx_plan* p) {
     if (!p) return;
     void* ptrs[] = {p->twid, p->svec, p->sscal, p->sprob, p->sdat, p->q, p->qd, p->spec,
-                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage};
+                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft};
     for (void* x : ptrs)
         if (x) (void)hipFree(x);
     delete p;
@@ -1217,6 +1487,38 @@ extern "C" int vx_plan_info(const vx_plan* p, int64_t* info) {
     return VX_OK;
 }
 
+// (Re)build the MFMA x pass's DFT matrices for the plan's support box.
+static int ensure_dft(vx_plan* p) {
+    if (p->xvariant != 2 || (p->dft && p->dft_lo == p->lo && p->dft_w == p->w)) return VX_OK;
+    const int n = p->n, NP = 16 * ((n + 15) / 16);
+    const int KP = (p->w + 7) / 8 * 8;
+    std::vector<unsigned short> fh, fl, gh, gl;
+    build_dft_matrix(n, NP, KP, p->lo, p->w, true, fh, fl);
+    build_dft_matrix(n, NP, NP, 0, n, false, gh, gl);
+    const size_t fe = (size_t)2 * NP * 2 * NP;   // capacity: KP <= NP
+    if (!p->dft) {
+        if (hipMalloc((void**)&p->dft, 4 * fe * sizeof(unsigned short)) != hipSuccess) {
+            (void)hipGetLastError();
+            p->dft = nullptr;
+            set_error("device allocation of the DFT matrices failed");
+            return VX_ERR_NOMEM;
+        }
+        p->bytes += 4 * fe * sizeof(unsigned short);
+    }
+    p->dft_elems[0] = fe;
+    p->dft_elems[1] = fe;
+    VX_HIP(hipDeviceSynchronize());   // no launch in flight may still read the old matrices
+    unsigned short* d = p->dft;
+    VX_HIP(hipMemcpy(d, fh.data(), fh.size() * 2, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(d + fe, fl.data(), fl.size() * 2, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(d + 2 * fe, gh.data(), gh.size() * 2, hipMemcpyHostToDevice));
+    VX_HIP(hipMemcpy(d + 3 * fe, gl.data(), gl.size() * 2, hipMemcpyHostToDevice));
+    p->dft_lo = p->lo;
+    p->dft_w = p->w;
+    p->dft_kp = KP;
+    return VX_OK;
+}
+
 namespace vx {
 // Largest squared distance from the rotation centre n/2 of a voxel whose
 // density (or, in prob mode, any probability channel) is non-zero.
@@ -1288,7 +1590,7 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
         p->w = hi - lo + 1;
     }
     p->have_search = true;
-    return VX_OK;
+    return ensure_dft(p);
 }
 
 extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec, const float* tprob,
@@ -1329,7 +1631,13 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
     VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
     switch (key) {
         case VX_TUNE_XPASS_VARIANT:
-            VX_REQUIRE(value == 0 || value == 1, VX_ERR_ARG, "x-pass variant must be 0 or 1");
+            VX_REQUIRE(value >= 0 && value <= 2, VX_ERR_ARG, "x-pass variant must be 0, 1 or 2");
+            if (value == 2) {
+                VX_REQUIRE(p->G == 1, VX_ERR_UNSUPPORTED, "the MFMA x pass has one correlation group");
+                VX_REQUIRE((p->n + 15) / 16 <= kMfmaMaxNPT &&
+                           xcorr_mfma_lds(16 * ((p->n + 15) / 16), 16 * ((p->n + 15) / 16)) <= kMaxLds,
+                           VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", p->n);
+            }
             p->xvariant = value; break;
         case VX_TUNE_XPASS_TILE:
             VX_REQUIRE(value >= 0 && value <= 64, VX_ERR_ARG, "x-pass tile must be in [0, 64]");
@@ -1356,7 +1664,7 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
         default:
             VX_REQUIRE(false, VX_ERR_ARG, "unknown tuning key %d", key);
     }
-    return VX_OK;
+    return ensure_dft(p);
 }
 
 extern "C" int vx_set_prob_stats(vx_plan* p, int enable, double alpha, double vstd, double vave,
@@ -1404,8 +1712,16 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
                        : launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
         if (rc != VX_OK) return rc;
         VX_MARK(2);
-        rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
-                          p->xvariant, p->xT);
+        if (p->xvariant == 2) {
+            VX_REQUIRE(p->dft && p->dft_lo == p->lo && p->dft_w == p->w, VX_ERR_STATE,
+                       "MFMA x-pass matrices do not match the support box");
+            const unsigned short* mats[4] = {p->dft, p->dft + p->dft_elems[0], p->dft + 2 * p->dft_elems[0],
+                                             p->dft + 2 * p->dft_elems[0] + p->dft_elems[1]};
+            rc = launch_xcorr_mfma(p->spec, p->tgt, p->sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, st);
+        } else {
+            rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
+                              p->xvariant, p->xT);
+        }
         if (rc != VX_OK) return rc;
         VX_MARK(3);
         float* ro = (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr;
