@@ -55,6 +55,32 @@ def _stage_bytes(n: int, w: int, b: int) -> dict:
             "inv_yz": b * nh * 8}
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "xcorr_pair200_pmc.md")
+
+
+def pmc_traffic(workload: str, batch: int, schedule, box_width: int):
+    """HBM bytes per x-pass launch measured by rocprofv3 (FETCH_SIZE and
+    WRITE_SIZE passes, calibrated), read from the committed summary's line
+    ``traffic: workload=W batch=B variant=V tile=T box=W bytes=X``; None when
+    no line matches this run's workload, batch, x-pass schedule and box."""
+    if not isinstance(schedule, dict):
+        schedule = {"xpass_variant": 1, "xpass_tile": 8}
+    want = {"workload": workload, "batch": str(batch), "variant": str(schedule.get("xpass_variant")),
+            "tile": str(schedule.get("xpass_tile")), "box": str(box_width)}
+    try:
+        with open(PMC_SUMMARY) as f:
+            for line in f:
+                line = line.strip().strip("`")
+                if not line.startswith("traffic:"):
+                    continue
+                kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
+                if all(kv.get(k) == v for k, v in want.items()):
+                    return int(kv["bytes"])
+    except OSError:
+        pass
+    return None
+
+
 def prepare(workload: str):
     from vesper_python_amd import synth
     from vesper_python_amd import VESPER_1_prob_new as V
@@ -134,7 +160,7 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--rows", type=int, default=0, help="score only the first ROWS rows per step (profiling)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
-                    help="first run a device copy of this many MiB (counter calibration)")
+                    help="first run a fill and a scale of this many MiB (counter calibration)")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -149,9 +175,10 @@ def main():
 
     engine.require_gpu()
     if args.calib_copy_mb:
+        # counter calibration: a fill (writes S bytes) and an elementwise
+        # scale (reads S, writes S) of S = calib_copy_mb MiB
         src = torch.ones(args.calib_copy_mb * 262144, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
+        dst = src * 2.0
         torch.cuda.synchronize()
         del src, dst
     w, n1, n2 = prepare(args.workload)
@@ -216,6 +243,10 @@ def main():
                 "support_box": [info["box_lo"], info["box_width"]],
                 "stage_GBps": {k: round(sb[k] / (stage_ms[k] / launches / 1e3) / 1e9, 1) for k in sb}}
 
+    if dom == "xcorr":
+        roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
+                                          info["box_width"])
+
     proposal = None
     if not args.no_proposal and not args.rows:
         proposal = measure_proposal(plan, n1, n2, w["A"], best, R, value)
@@ -226,6 +257,8 @@ def main():
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
            "data": "synthetic", "top10_recall": 1.0,
+           "top10_recall_basis": "value is the exhaustive search itself; the learned proposal's recall "
+                                 "against it is proposal.recall_top10",
            "config": {"workload": f"{args.workload}: {w['text']}", "grid_edge": n, "rows": R,
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},

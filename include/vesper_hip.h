@@ -196,6 +196,46 @@ int vx_threshold_dmax(float* data_dev, int sx, int sy, int sz, float th, double*
 /* laplacian_filter (VESPER_1_prob_new.py:494-526) on an [n][n][n] float64 grid. */
 int vx_laplacian(int n, const double* in_dev, double* out_dev, vx_stream stream);
 
+/*
+ * fft_search_best_dot (VESPER_1_prob_new.py:464-491) and the transform of
+ * fft_search_score_trans_1d (:529-565): for each of nch channels,
+ * out_dev[c] = irfftn(tspec_dev[c] * rfftn(q_dev[c])), unnormalised, with the
+ * caller's conjugated target spectra tspec_dev (nch x [n][n][n/2+1]
+ * complex64, the layout of numpy's rfftn), queries q_dev (nch x [n]^3
+ * float32) and outputs out_dev (nch x [n]^3 float32).  Uses the plan's FFT
+ * workspace (any mode; the plan's grid edge); does not touch its target or
+ * query.
+ */
+int vx_correlate(vx_plan* plan, const void* tspec_dev, int nch, const float* q_dev, float* out_dev,
+                 vx_stream stream);
+
+/*
+ * find_best_trans_list (VESPER_1_prob_new.py:357-374): the float32 channel
+ * sum of nmaps (<= 8) real correlation maps of n3 voxels each, its maximum and
+ * first (C-order) argmax.  maps_host holds nmaps device pointers.  Writes
+ * best_dev[0] and index_dev[0] (flat index; the int64 doubles as scratch).
+ */
+int vx_sum_argmax(const float* const* maps_host, int nmaps, int64_t n3, float* best_dev,
+                  int64_t* index_dev, vx_stream stream);
+
+/*
+ * find_best_trans_mixed (VESPER_1_prob_new.py:1530-1545): Python-sum of each
+ * group, float32 z-scores (vave, vstd, pave, pstd rounded to float32) and the
+ * (1 - alpha), alpha mix; maximum and first argmax as vx_sum_argmax.
+ */
+int vx_mixed_argmax(const float* const* vmaps_host, int nv, const float* const* pmaps_host, int np,
+                    int64_t n3, double alpha, double vstd, double vave, double pstd, double pave,
+                    float* best_dev, int64_t* index_dev, vx_stream stream);
+
+/*
+ * sco_arr of get_score (VESPER_1_prob_new.py:1362-1370): float32 per-voxel
+ * products tvec . svec placed at the search positions t + trans (trans
+ * wrapped as the reference does), zero elsewhere; out_dev is [n]^3 float64
+ * (out_f64 = 1, the orig mode's dtype) or float32 (prob mode).
+ */
+int vx_sco_arr(int n, const float* tvec_dev, const float* svec_dev, const int32_t* trans_host3,
+               int out_f64, void* out_dev, vx_stream stream);
+
 
 /* ------------------------------------------------------------------------
  * Learned rotation proposal.  No counterpart in the reference (which scores

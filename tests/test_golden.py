@@ -1,0 +1,69 @@
+"""Committed golden fixtures (tests/golden/, written by make_golden.py) on the
+CPU: the known-answer tables against closed forms and the package's host
+geometry, and the oracle against its own frozen outputs (a change to the
+oracle that moves any fixture fails here)."""
+import json
+import os
+
+import numpy as np
+
+from oracle import vesper_ref as ref
+from oracle.build_c import fastvec_c
+from vesper_python_amd import geometry
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _load(name):
+    return np.load(os.path.join(GOLD, name))
+
+
+def test_known_answer_tables():
+    with open(os.path.join(GOLD, "known_answers.json")) as f:
+        k = json.load(f)
+    fast = k["next_fast_len_1_to_400"]
+    assert [geometry.next_fast_len(n) for n in range(1, 401)] == fast
+    # 39 and 55 are grid edges the reference's notebooks print
+    assert 39 in fast and 55 in fast and fast[33] == 35 and fast[36] == 39
+    # x, y in [0, 360) and z in [0, 180] by repeated addition of A
+    for a, r in k["angle_rows"].items():
+        a = float(a)
+        nx = len(np.arange(0, 360, a))
+        nz = len(np.arange(0, 180 + 1e-9, a))
+        assert r == nx * nx * nz == len(geometry.calc_angle_comb(a))
+    assert k["angle_rows"] == {"30.0": 1008, "10.0": 24624, "5.0": 191808}
+    assert k["refine_rows_for_angle_0_0_0"] == 27 == len(geometry.refine_rows([0.0, 0.0, 0.0]))
+
+
+def test_host_rotation_rows_match_fixture():
+    g = _load("angle_rows_a30.npz")
+    assert np.array_equal(geometry.calc_angle_comb(30.0), g["angles"])
+    assert np.array_equal(geometry.rotation_rows(g["angles"]), g["flip_f32"])
+
+
+def test_oracle_reproduces_rot_mrc_fixtures():
+    for n in (9, 10):
+        g = _load(f"rot_mrc_n{n}.npz")
+        for i, a in enumerate(g["angles"]):
+            v, d = ref.rot_mrc(g["data"], g["vec"], a, fma=True)
+            assert np.array_equal(d, g["out_data"][i]) and np.array_equal(v, g["out_vec"][i])
+
+
+def test_oracle_reproduces_search_fixture():
+    g = _load("search_VecProduct_n12.npz")
+    st = ref.map_stats(g["tdata"])
+    t = ref.MapGrid(data=g["tdata"], xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=g["tvec"],
+                    ave=st["ave"], std=st["std"], std_norm_ave=st["std_norm_ave"])
+    spec = ref.target_spectra(ref.target_channels(t, "VecProduct"))
+    for i, a in enumerate(g["angles"][:3]):
+        nv, nd = ref.rot_mrc(g["sdata"], g["svec"], a, prec="ref64")
+        b, tr = ref.find_best_trans_list(ref.fft_correlate(spec, ref.query_channels(nd, nv, "VecProduct"), "ref64"))
+        assert float(b) == g["best64"][i] and tuple(tr) == tuple(g["trans64"][i])
+
+
+def test_c_oracle_reproduces_fastvec_fixture(fastvec_ref):
+    g = _load("fastvec_n9.npz")
+    d, v = fastvec_c(fastvec_ref, g["src"], float(g["src_width"]), g["src_orig"], int(g["n"]),
+                     float(g["dest_width"]), g["dest_orig"], float(g["dreso"]))
+    assert np.array_equal(d, g["data"]) and np.array_equal(v, g["vec"])
+    assert (d != 0).sum() > 10

@@ -374,3 +374,50 @@ def test_mfma_xpass_agrees_with_fft_at_n96(gpu):
     rel = np.abs(bm - bf) / np.abs(bf).max()
     print("[mfma] max rel diff", rel.max(), "trans agree", np.mean(np.all(tf == tm, axis=1)), flush=True)
     assert rel.max() <= RTOL
+
+
+# ------------------------------------------- per-channel correlation helpers
+
+@pytest.mark.parametrize("n", [12, 15, 20])
+def test_fft_search_best_dot_matches_oracle(gpu, n):
+    """fft_search_best_dot (VESPER_1_prob_new.py:464-491) through
+    vx_correlate: every channel's correlation map within 1e-4 of the FP64
+    oracle's largest value."""
+    from vesper_python_amd import VESPER_1_prob_new as V
+    rng = np.random.default_rng(800 + n)
+    td, tv = _maps(rng, n, 0.4)
+    sd, sv = _maps(rng, n, 0.4)
+    spec = ref.target_spectra([tv[..., k] for k in range(3)])
+    q = [sv[..., k] for k in range(3)]
+    _say("best_dot", n)
+    got = V.fft_search_best_dot(spec, q, None, None, None, None, None)
+    e64 = ref.fft_correlate(spec, q, "ref64")
+    for g, e in zip(got, e64):
+        assert g.dtype == np.float32 and g.shape == (n, n, n)
+        assert np.max(np.abs(g - e)) <= RTOL * np.max(np.abs(e))
+    b, t = V.find_best_trans_list(got)
+    eb, et = ref.find_best_trans_list(e64)
+    s64 = sum(e64)
+    assert abs(b - eb) <= RTOL * abs(eb)
+    assert t == et or abs(s64[t] - eb) <= RTOL * abs(eb)
+
+
+@pytest.mark.parametrize("mode", ["Overlap", "CC", "PCC", "Laplacian"])
+def test_fft_search_score_trans_1d_modes(gpu, mode):
+    from vesper_python_amd import VESPER_1_prob_new as V
+    n = 14
+    rng = np.random.default_rng(850)
+    td, _ = _maps(rng, n, 0.4)
+    sd, _ = _maps(rng, n, 0.4)
+    ave = 0.37
+    tch = {"Overlap": np.where(td > 0, 1.0, 0.0), "CC": np.where(td > 0, td, 0.0),
+           "PCC": np.where(td > 0, td - ave, 0.0), "Laplacian": ref.laplacian_filter(td)}[mode]
+    spec = ref.target_spectra([tch])
+    qch = {"Overlap": np.where(sd > 0, 1.0, 0.0), "CC": np.where(sd > 0, sd, 0.0),
+           "PCC": np.where(sd > 0, sd - ave, 0.0), "Laplacian": ref.laplacian_filter(sd)}[mode]
+    _say("score_trans_1d", mode)
+    b, t = V.fft_search_score_trans_1d(spec[0], sd, None, None, None, None, mode, ave)
+    e = ref.fft_correlate(spec, [qch.astype(np.float32)], "ref64")[0]
+    eb = float(e.max())
+    assert abs(b - eb) <= RTOL * abs(eb)
+    assert abs(e[t] - eb) <= RTOL * abs(eb)

@@ -142,3 +142,67 @@ def test_cli_prob_runs(gpu, prob_pair):
                         "-npb", paths["B_prob"], "-t", str(p.a.threshold), "-T", str(p.b.threshold),
                         "-s", "7", "-A", "30", "-N", "3", "-alpha", "0.5"])
     assert len(res) == 3 and "###Start Refining###" in buf.getvalue()
+
+
+class _Grid:
+    def __init__(self, data, vec=None):
+        self.data, self.vec, self.xdim = data, vec, data.shape[0]
+
+
+def test_rot_and_search_fft_prob_dropin(gpu):
+    """The drop-in per-row prob function (VESPER_1_prob_new.py:1462-1527),
+    without and with the mixing statistics, against the oracle."""
+    n = 14
+    rng = np.random.default_rng(660)
+    td, tv, tp = _maps(rng, n)
+    sd, sv, sp = _maps(rng, n)
+    spec = ref.target_spectra([tv[..., 0], tv[..., 1], tv[..., 2]] + tp)
+    tl = {"target": _Grid(td, tv), "P1": _Grid(tp[0]), "P2": _Grid(tp[1]), "P3": _Grid(tp[2]),
+          "P4": _Grid(tp[3])}
+    alpha, stats = 0.3, (2.0, 0.1, 4.0, 0.2)
+    for ang in ([0, 0, 0], [40, 110, 70]):
+        print("[case] rot_and_search_fft_prob", ang, flush=True)
+        g = V.rot_and_search_fft_prob(sd, sv, *sp, ang, tl, alpha)
+        assert g[4] is None and g[5] is None
+        gm = V.rot_and_search_fft_prob(sd, sv, *sp, ang, tl, alpha, *stats)
+        e = ref.rot_and_search_fft_prob(sd, sv, sp, ang, spec, alpha, stats, prec="ref64")
+        rv = sum(ref.fft_correlate(spec[:3], [e[6][..., k] for k in range(3)], "ref64"))
+        _near(g[0], g[1], e[0], e[1], rv, "vec")
+        assert g[:4] == gm[:4]
+        assert abs(gm[4] - e[4]) <= RTOL * max(abs(e[4]), 1e-30) and gm[5] == tuple(e[5])
+        assert abs(g[2] - e[2]) <= RTOL * max(abs(e[2]), 1e-30)
+
+
+def test_best_trans_helpers_match_oracle(gpu):
+    """find_best_trans_list / find_best_trans_mixed on caller-held maps: the
+    float32 sums, z-scores and first argmax bit-exact against the oracle."""
+    rng = np.random.default_rng(670)
+    n = 12
+    vm = [rng.standard_normal((n, n, n)).astype(np.float32) * 50 for _ in range(3)]
+    pm = [rng.uniform(0, 30, (n, n, n)).astype(np.float32) for _ in range(4)]
+    vm[0][3, 4, 5] = 500.0                               # a clear maximum
+    tie = [np.ones((n, n, n), np.float32) for _ in range(2)]
+    for maps in (vm, pm, tie):
+        print("[case] sum_argmax", len(maps), flush=True)
+        assert V.find_best_trans_list(maps) == ref.find_best_trans_list(maps)
+    for args in ((0.5, 3.0, 0.2, 5.0, 0.1), (0.0, 1.0, 0.0, 1.0, 0.0), (1.0, 7.3, -1.1, 2.2, 0.4)):
+        print("[case] mixed_argmax", args, flush=True)
+        g = V.find_best_trans_mixed(vm, pm, *args)
+        e = ref.find_best_trans_mixed(vm, pm, *args, prec="ref32")
+        assert g[0] == e[0] and tuple(g[1]) == tuple(e[1]), (g, e)
+
+
+def test_sco_arr_matches_oracle(gpu):
+    n = 13
+    rng = np.random.default_rng(680)
+    td, tv, _ = _maps(rng, n, 0.5)
+    sd, sv, _ = _maps(rng, n, 0.5)
+    st = ref.map_stats(td)
+    t = ref.MapGrid(data=td, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=tv,
+                    ave=st["ave"], std=st["std"], std_norm_ave=st["std_norm_ave"])
+    for trans in ((0, 0, 0), (2, 11, 5), (12, 1, 7)):
+        print("[case] sco_arr", trans, flush=True)
+        g = engine.sco_arr(engine.to_dev(tv, torch.float32), engine.to_dev(sv, torch.float32), trans).cpu().numpy()
+        e = ref.get_score(t, sd, sv, trans, "ref32")["sco_arr"]
+        print("[sco_arr] exact fraction", np.mean(g == e), flush=True)
+        assert np.allclose(g, e, rtol=1e-6, atol=1e-7)

@@ -329,15 +329,9 @@ def get_score(target_map, search_map_data, search_map_vec, trans, _dev=None):
 
 def _sco_arr(tvec, svec, trans, dtype):
     """Per-voxel dot products placed at the search positions (sco_arr of
-    get_score, VESPER_1_prob_new.py:1375-1377), for the PDB B-factors."""
-    n = int(tvec.shape[0])
-    t = [int(v) - n if int(v) > 0.5 * n else int(v) for v in trans]
-    out = torch.zeros((n, n, n), dtype=torch.float64, device=tvec.device)
-    xs = [slice(max(0, -t[a]), min(n, n - t[a])) for a in range(3)]
-    ys = [slice(xs[a].start + t[a], xs[a].stop + t[a]) for a in range(3)]
-    prod = (tvec[xs[0], xs[1], xs[2]] * svec[ys[0], ys[1], ys[2]]).sum(dim=-1)
-    out[ys[0], ys[1], ys[2]] = prod.double()
-    return out.cpu().numpy().astype(dtype, copy=False)
+    get_score, VESPER_1_prob_new.py:1362-1370), for the PDB B-factors."""
+    out = engine.sco_arr(tvec, svec, trans, torch.float64 if np.dtype(dtype) == np.float64 else torch.float32)
+    return out.cpu().numpy()
 
 
 def show_vec(origin, sampled_mrc_vec, sampled_mrc_data, score_arr, score, sample_width, trans, angle,
@@ -390,33 +384,69 @@ def rot_and_search_fft(data, vec, angle, target_list, mrc_target, mode="VecProdu
     return score, _trans_tuple(trans[0]), ov.cpu().numpy(), od.cpu().numpy().astype(np.asarray(data).dtype)
 
 
+_CORR_PLANS = {}
+
+
+def _corr_plan(n):
+    """A small plan per grid edge for the per-channel correlation helpers."""
+    p = _CORR_PLANS.get(n)
+    if p is None:
+        p = engine.SearchPlan(n, nat.VX_MODE_CC, batch=1)
+        _CORR_PLANS[n] = p
+    return p
+
+
+def fft_search_best_dot(target_list, query_list, a=None, b=None, c=None, fft_object=None, ifft_object=None):
+    """VESPER_1_prob_new.py:464-491 on the device (vx_correlate): the
+    unnormalised correlation map irfftn(target * rfftn(query)) per channel.
+    ``target_list`` holds conjugated target spectra ([n, n, n/2+1] complex);
+    the pyFFTW buffers and plans ``a, b, c, fft_object, ifft_object`` are
+    accepted and ignored."""
+    q = np.stack([np.asarray(x, dtype=np.float32) for x in query_list])
+    n = q.shape[1]
+    ts = torch.from_numpy(np.stack([np.asarray(t).astype(np.complex64) for t in target_list]))
+    out = _corr_plan(n).correlate(ts.to(engine.require_gpu()), engine.to_dev(q, torch.float32))
+    host = out.cpu().numpy()
+    return [host[i] for i in range(host.shape[0])]
+
+
+def fft_search_score_trans_1d(target_X, search_data, a=None, b=None, fft_object=None, ifft_object=None,
+                              mode="CC", ave=None):
+    """VESPER_1_prob_new.py:529-565: the mode's query transform (Overlap,
+    CC, PCC on the host as the reference does; Laplacian with vx_laplacian),
+    the correlation with ``target_X`` (vx_correlate) and its best
+    translation (vx_sum_argmax)."""
+    x2 = copy.deepcopy(np.asarray(search_data))
+    if mode == "Overlap":
+        x2 = np.where(x2 > 0, 1.0, 0.0)
+    elif mode == "CC":
+        x2 = np.where(x2 > 0, x2, 0.0)
+    elif mode == "PCC":
+        x2 = np.where(x2 > 0, x2 - ave, 0.0)
+    elif mode == "Laplacian":
+        x2 = laplacian_filter(x2)
+    n = x2.shape[0]
+    ts = torch.from_numpy(np.asarray(target_X).astype(np.complex64)[None])
+    q = engine.to_dev(np.asarray(x2, dtype=np.float32)[None], torch.float32)
+    out = _corr_plan(n).correlate(ts.to(q.device), q)
+    best, i = engine.sum_argmax([out[0]])
+    return best, tuple(int(v) for v in np.unravel_index(i, (n, n, n)))
+
+
 def find_best_trans_list(input_list):
     """Channel sum, max and first argmax (VESPER_1_prob_new.py:357-374),
-    reduced on the device."""
+    reduced on the device (vx_sum_argmax)."""
     arrs = [engine.to_dev(np.asarray(a, dtype=np.float32), torch.float32) for a in input_list]
-    s = arrs[0].clone()
-    for a in arrs[1:]:
-        s = s + a
-    return _device_argmax(s)
-
-
-def _device_argmax(s: torch.Tensor):
-    flat = s.reshape(-1)
-    rows = engine.topn(flat, 1)
-    i = int(rows[0])
-    best = np.float32(flat[i].item())
-    return best, tuple(int(v) for v in np.unravel_index(i, tuple(s.shape)))
+    best, i = engine.sum_argmax(arrs)
+    return best, tuple(int(v) for v in np.unravel_index(i, tuple(arrs[0].shape)))
 
 
 def find_best_trans_mixed(vec_fft_results, prob_fft_results, alpha, vstd, vave, pstd, pave):
-    """VESPER_1_prob_new.py:1530-1545 in float32 on the device."""
-    f32 = np.float32
-    sv = sum(engine.to_dev(np.asarray(a, dtype=np.float32), torch.float32) for a in vec_fft_results)
-    sp = sum(engine.to_dev(np.asarray(a, dtype=np.float32), torch.float32) for a in prob_fft_results)
-    sv = (sv - float(f32(vave))) / float(f32(vstd))
-    sp = (sp - float(f32(pave))) / float(f32(pstd))
-    mixed = float(f32(1 - alpha)) * sv + float(f32(alpha)) * sp
-    return _device_argmax(mixed)
+    """VESPER_1_prob_new.py:1530-1545 in float32 on the device (vx_mixed_argmax)."""
+    v = [engine.to_dev(np.asarray(a, dtype=np.float32), torch.float32) for a in vec_fft_results]
+    p = [engine.to_dev(np.asarray(a, dtype=np.float32), torch.float32) for a in prob_fft_results]
+    best, i = engine.mixed_argmax(v, p, alpha, vstd, vave, pstd, pave)
+    return best, tuple(int(x) for x in np.unravel_index(i, tuple(v[0].shape)))
 
 
 def search_map_fft_prob(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, mrc_search_p1, mrc_search_p2,

@@ -29,7 +29,7 @@ struct FftStage {
     int R;          // radix
     int P;          // product of the earlier radices
     float Pinv;     // 1 / P, for the in-kernel k = i mod P
-    int tw;         // float2 offset of the stage's twiddles
+    int tw;         // float2 offset of the stage's twiddles, laid out [j - 1][k], j < R, k < P
 };
 
 struct FftDesc {
@@ -201,9 +201,9 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
             for (int j = 0; j < R; ++j) v[m][j] = L[(i + j * T) * es];
             const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
             if (k != 0) {
-                const float2* w = tw + k * (R - 1);
+                const float2* w = tw + k;
 #pragma unroll
-                for (int j = 1; j < R; ++j) v[m][j] = cmul(v[m][j], w[j - 1]);
+                for (int j = 1; j < R; ++j) v[m][j] = cmul(v[m][j], w[(j - 1) * p]);
             }
             dft<R>(v[m]);
             obase[m] = line * ls + ((i - k) * R + k) * es;
@@ -259,9 +259,9 @@ __device__ __forceinline__ void butterfly_pp(const float2* __restrict__ in, floa
     for (int j = 0; j < R; ++j) v[j] = L[(i + j * T) * es];
     const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
     if (k != 0) {
-        const float2* w = tw + k * (R - 1);
+        const float2* w = tw + k;
 #pragma unroll
-        for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[j - 1]);
+        for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[(j - 1) * p]);
     }
     dft<R>(v);
     float2* O = out + line * ls + ((i - k) * R + k) * es;

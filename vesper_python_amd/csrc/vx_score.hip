@@ -2,6 +2,7 @@
 // and the C-ABI's error/version entry points.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <algorithm>
 #include <mutex>
 #include <string.h>
 
@@ -399,5 +400,156 @@ extern "C" int vx_threshold_dmax(float* data, int sx, int sy, int sz, float th, 
     VX_REQUIRE(h[1] > 0, VX_ERR_ARG, "no voxel above the threshold");
     out2[0] = h[0];
     out2[1] = h[1];
+    return VX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Reductions over caller-held real correlation maps (the per-row helpers of
+// the reference module) and get_score's per-voxel products.
+// ---------------------------------------------------------------------------
+
+namespace vx {
+
+constexpr int kMaxMaps = 8;
+struct MapSet {
+    const float* p[kMaxMaps];
+};
+
+// find_best_trans_list (VESPER_1_prob_new.py:365-369): sum_arr = 0; sum_arr =
+// sum_arr + arr per channel in float32; amax and the first argmax.
+__global__ void __launch_bounds__(256)
+k_sum_argmax(MapSet m, int nm, size_t n3, unsigned long long* __restrict__ key) {
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int c = 0; c < nm; ++c) s = __fadd_rn(s, m.p[c][i]);
+        if (better(s, (uint32_t)i, bv, bi)) { bv = s; bi = (uint32_t)i; }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) atomicMax(key, argmax_key(bv, bi));
+}
+
+// find_best_trans_mixed (:1530-1545): Python sum() of each group, float32
+// z-scores with float32-rounded constants, alpha mix; no contraction.
+__global__ void __launch_bounds__(256)
+k_mixed_argmax(MapSet v, int nv, MapSet p, int np, size_t n3, float vave, float vstd, float pave, float pstd,
+               float wa, float wb, unsigned long long* __restrict__ key) {
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        float a = 0.f, b = 0.f;
+        for (int c = 0; c < nv; ++c) a = __fadd_rn(a, v.p[c][i]);
+        for (int c = 0; c < np; ++c) b = __fadd_rn(b, p.p[c][i]);
+        const float sv = __fdiv_rn(__fsub_rn(a, vave), vstd);
+        const float sp = __fdiv_rn(__fsub_rn(b, pave), pstd);
+        const float mx = __fadd_rn(__fmul_rn(wa, sv), __fmul_rn(wb, sp));
+        if (better(mx, (uint32_t)i, bv, bi)) { bv = mx; bi = (uint32_t)i; }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) atomicMax(key, argmax_key(bv, bi));
+}
+
+// key (held in the int64 output) -> best value and flat C-order index
+__global__ void k_key_out(float* __restrict__ best, int64_t* __restrict__ index) {
+    if (threadIdx.x != 0) return;
+    const unsigned long long k = (unsigned long long)index[0];
+    best[0] = unordered_bits((uint32_t)(k >> 32));
+    index[0] = (int64_t)(0xffffffffu - (uint32_t)(k & 0xffffffffu));
+}
+
+__global__ void k_zero_u64(int64_t* p) {
+    if (threadIdx.x == 0) p[0] = 0;
+}
+
+// sco_arr (:1362-1370): at each in-range search position s = t + trans, the
+// float32 product sum tv . sv, accumulated as ((0 + x) + y) + z.
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_sco_arr(const float* __restrict__ tvec, const float* __restrict__ svec, int n, int t0, int t1, int t2,
+          T* __restrict__ out) {
+    const size_t n3 = (size_t)n * n * n;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n3; i += (size_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i % n), y = (int)((i / n) % n), x = (int)(i / ((size_t)n * n));
+        const int sx = x + t0, sy = y + t1, sz = z + t2;
+        if (sx < 0 || sy < 0 || sz < 0 || sx >= n || sy >= n || sz >= n) continue;
+        const size_t j = ((size_t)sx * n + sy) * n + sz;
+        float a = 0.f;
+        a = __fadd_rn(a, __fmul_rn(tvec[3 * i + 0], svec[3 * j + 0]));
+        a = __fadd_rn(a, __fmul_rn(tvec[3 * i + 1], svec[3 * j + 1]));
+        a = __fadd_rn(a, __fmul_rn(tvec[3 * i + 2], svec[3 * j + 2]));
+        out[j] = (T)a;
+    }
+}
+
+}  // namespace vx
+
+static int fill_mapset(vx::MapSet* s, const float* const* maps, int nm) {
+    VX_REQUIRE(maps != nullptr && nm >= 1 && nm <= vx::kMaxMaps, VX_ERR_ARG, "need 1..%d maps", vx::kMaxMaps);
+    for (int c = 0; c < vx::kMaxMaps; ++c) s->p[c] = c < nm ? maps[c] : nullptr;
+    for (int c = 0; c < nm; ++c) VX_REQUIRE(maps[c] != nullptr, VX_ERR_ARG, "map %d is NULL", c);
+    return VX_OK;
+}
+
+extern "C" int vx_sum_argmax(const float* const* maps, int nmaps, int64_t n3, float* best, int64_t* index,
+                             vx_stream stream) {
+    VX_REQUIRE(best && index && n3 > 0 && n3 < 0xffffffffLL, VX_ERR_ARG, "bad arguments");
+    vx::MapSet m;
+    int rc = fill_mapset(&m, maps, nmaps);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(vx::k_zero_u64, dim3(1), dim3(64), 0, st, index);
+    VX_LAUNCH_CHECK();
+    const unsigned g = (unsigned)std::min<int64_t>((n3 + 255) / 256, 2048);
+    hipLaunchKernelGGL(vx::k_sum_argmax, dim3(g), dim3(256), 0, st, m, nmaps, (size_t)n3,
+                       (unsigned long long*)index);
+    VX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(vx::k_key_out, dim3(1), dim3(64), 0, st, best, index);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+extern "C" int vx_mixed_argmax(const float* const* vmaps, int nv, const float* const* pmaps, int np, int64_t n3,
+                               double alpha, double vstd, double vave, double pstd, double pave, float* best,
+                               int64_t* index, vx_stream stream) {
+    VX_REQUIRE(best && index && n3 > 0 && n3 < 0xffffffffLL, VX_ERR_ARG, "bad arguments");
+    vx::MapSet v, p;
+    int rc = fill_mapset(&v, vmaps, nv);
+    if (rc) return rc;
+    rc = fill_mapset(&p, pmaps, np);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(vx::k_zero_u64, dim3(1), dim3(64), 0, st, index);
+    VX_LAUNCH_CHECK();
+    const unsigned g = (unsigned)std::min<int64_t>((n3 + 255) / 256, 2048);
+    hipLaunchKernelGGL(vx::k_mixed_argmax, dim3(g), dim3(256), 0, st, v, nv, p, np, (size_t)n3, (float)vave,
+                       (float)vstd, (float)pave, (float)pstd, (float)(1.0 - alpha), (float)alpha,
+                       (unsigned long long*)index);
+    VX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(vx::k_key_out, dim3(1), dim3(64), 0, st, best, index);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
+}
+
+extern "C" int vx_sco_arr(int n, const float* tvec, const float* svec, const int32_t* trans, int out_f64,
+                          void* out, vx_stream stream) {
+    VX_REQUIRE(n > 0 && tvec && svec && trans && out, VX_ERR_ARG, "bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    const size_t n3 = (size_t)n * n * n;
+    int t[3];
+    for (int a = 0; a < 3; ++a) t[a] = trans[a] > 0.5 * n ? trans[a] - n : trans[a];
+    VX_HIP(hipMemsetAsync(out, 0, n3 * (out_f64 ? sizeof(double) : sizeof(float)), st));
+    const unsigned g = (unsigned)std::min<size_t>((n3 + 255) / 256, 4096);
+    if (out_f64)
+        hipLaunchKernelGGL(vx::k_sco_arr<double>, dim3(g), dim3(256), 0, st, tvec, svec, n, t[0], t[1], t[2],
+                           (double*)out);
+    else
+        hipLaunchKernelGGL(vx::k_sco_arr<float>, dim3(g), dim3(256), 0, st, tvec, svec, n, t[0], t[1], t[2],
+                           (float*)out);
+    VX_LAUNCH_CHECK();
     return VX_OK;
 }

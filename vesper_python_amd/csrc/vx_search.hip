@@ -63,8 +63,9 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
         stages[s].P = p;
         stages[s].Pinv = 1.0f / (float)p;
         stages[s].tw = (int)tw.size();
-        for (int k = 0; k < p; ++k)
-            for (int j = 1; j < R; ++j) {
+        // [j - 1][k]: lanes holding consecutive k read consecutive twiddles
+        for (int j = 1; j < R; ++j)
+            for (int k = 0; k < p; ++k) {
                 const double a = -2.0 * M_PI * (double)j * (double)k / ((double)p * (double)R);
                 tw.push_back(make_float2((float)cos(a), (float)sin(a)));
             }
@@ -464,6 +465,20 @@ k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2*
 // laid out so that the B workgroups of one tile share an XCD (id mod 8) and
 // are dispatched together: the target tile is read from HBM once per batch
 // and from that XCD's L2 by the other rotations.
+// Workgroup -> (column tile, rotation) of the x pass.  Workgroups are handed
+// to the 8 XCDs round-robin by linear id, so XCD k = id % 8 is given the
+// contiguous tile range [k tpx, (k + 1) tpx) and, within it, every rotation
+// of a tile back to back: neighbouring tiles share 128-byte lines of the
+// spectra and the target tile is reused by all rotations from that XCD's L2.
+__device__ __forceinline__ bool xcd_tile(int id, int nb, int ntiles, int* tile, int* b) {
+    const int tpx = (ntiles + 7) >> 3;
+    const int j = id >> 3;
+    const int tl = j / nb;
+    *b = j - tl * nb;
+    *tile = (id & 7) * tpx + tl;
+    return tl < tpx && *tile < ntiles;
+}
+
 template <int MAXE, int G>
 __global__ void __launch_bounds__(256)
 k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
@@ -473,10 +488,8 @@ k_xcorr_pp(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     float2* Bf = lds + n * T;
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
     const int id = blockIdx.x;
-    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
-    const int b = rem >> 3;
-    const int tile = grp * 8 + (rem & 7);
-    if (tile >= ntiles) return;
+    int b, tile;
+    if (!xcd_tile(id, nb, ntiles, &tile, &b)) return;
     const int plane = n * nhz;
     const int j0 = tile * T;
     const int nt = min(T, plane - j0);
@@ -569,10 +582,8 @@ k_xcorr_lds(const float2* __restrict__ spec, const float2* __restrict__ tgt, flo
             FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
     extern __shared__ float2 lds[];
     const int id = blockIdx.x;
-    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
-    const int b = rem >> 3;
-    const int tile = grp * 8 + (rem & 7);
-    if (tile >= ntiles) return;
+    int b, tile;
+    if (!xcd_tile(id, nb, ntiles, &tile, &b)) return;
     float2* A = lds;
     float2* Bf = lds + n * T;
     float2* acc = lds + 2 * n * T;                 // [G][n * T]
@@ -713,10 +724,8 @@ k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, fl
     constexpr int NP2 = 2 * NP;
     extern __shared__ unsigned short sm[];
     const int id = blockIdx.x;
-    const int grp = id / (8 * nb), rem = id - grp * 8 * nb;
-    const int b = rem >> 3;
-    const int tile = grp * 8 + (rem & 7);
-    if (tile >= ntiles) return;
+    int b, tile;
+    if (!xcd_tile(id, nb, ntiles, &tile, &b)) return;
     const int plane = n * nhz;
     const int j0 = tile * kMfmaCols;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1758,6 +1767,32 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
 extern "C" int vx_score_rows(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
                              vx_stream stream) {
     return score_rows_impl(p, rot, nrows, best, trans, (hipStream_t)stream, nullptr, nullptr);
+}
+
+// Per-channel correlation maps against caller-held conjugated spectra:
+// out_c = irfftn(tspec_c * rfftn(q_c)) unnormalised, for each channel c.
+extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float* q, float* out,
+                            vx_stream stream) {
+    VX_REQUIRE(p && tspec && q && out && nch >= 1, VX_ERR_ARG, "bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = p->n;
+    const size_t n3 = (size_t)n * n * n;
+    const size_t nh = (size_t)n * n * p->nhz;
+    const float2* ts = (const float2*)tspec;
+    const int xv = p->xvariant == 2 ? 1 : p->xvariant;
+    for (int c = 0; c < nch; ++c) {
+        const float* qc = q + (size_t)c * n3;
+        int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st)
+                           : launch_fwd_zy(qc, p->spec, p->fd, n, 1, 0, n, st);
+        if (rc != VX_OK) return rc;
+        rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT);
+        if (rc != VX_OK) return rc;
+        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st)
+                       : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st);
+        if (rc != VX_OK) return rc;
+    }
+    // the q planes were not touched, but the keys were: a later search re-zeroes them per batch
+    return VX_OK;
 }
 
 extern "C" int vx_score_rows_profile(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,

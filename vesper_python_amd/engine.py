@@ -99,6 +99,45 @@ def get_score(tdata, tvec, t_ave, sdata, svec, trans) -> dict:
                 ave2=out[5], std2=out[6], pstd2=out[7])
 
 
+def _ptrs(tensors):
+    arr = (ctypes.c_void_p * len(tensors))(*[t.data_ptr() for t in tensors])
+    return arr, ctypes.cast(arr, ctypes.c_void_p)
+
+
+def sum_argmax(maps) -> tuple:
+    """find_best_trans_list on the device: (float32 best, flat index) of the
+    float32 channel sum of `maps` (device float32 tensors of one shape)."""
+    maps = [m.contiguous() for m in maps]
+    keep, ptr = _ptrs(maps)
+    best = torch.empty(1, dtype=torch.float32, device=maps[0].device)
+    idx = torch.empty(1, dtype=torch.int64, device=maps[0].device)
+    nat.call("vx_sum_argmax", ptr, len(maps), int(maps[0].numel()), _p(best), _p(idx), _stream())
+    return np.float32(best.item()), int(idx.item())
+
+
+def mixed_argmax(vmaps, pmaps, alpha, vstd, vave, pstd, pave) -> tuple:
+    """find_best_trans_mixed on the device: (float32 best, flat index)."""
+    vmaps = [m.contiguous() for m in vmaps]
+    pmaps = [m.contiguous() for m in pmaps]
+    kv, pv = _ptrs(vmaps)
+    kp, pp = _ptrs(pmaps)
+    best = torch.empty(1, dtype=torch.float32, device=vmaps[0].device)
+    idx = torch.empty(1, dtype=torch.int64, device=vmaps[0].device)
+    nat.call("vx_mixed_argmax", pv, len(vmaps), pp, len(pmaps), int(vmaps[0].numel()), float(alpha),
+             float(vstd), float(vave), float(pstd), float(pave), _p(best), _p(idx), _stream())
+    return np.float32(best.item()), int(idx.item())
+
+
+def sco_arr(tvec: torch.Tensor, svec: torch.Tensor, trans, out_dtype=torch.float64) -> torch.Tensor:
+    """get_score's sco_arr (per-voxel float32 products at the search positions)."""
+    n = int(tvec.shape[0])
+    t = np.ascontiguousarray(np.asarray(trans, dtype=np.int32).reshape(3))
+    out = torch.empty((n, n, n), dtype=out_dtype, device=tvec.device)
+    nat.call("vx_sco_arr", n, _p(tvec), _p(svec), t.ctypes.data_as(ctypes.c_void_p),
+             int(out_dtype == torch.float64), _p(out), _stream())
+    return out
+
+
 def topn(scores: torch.Tensor, k: int, stride: int = 1, col: int = 0) -> np.ndarray:
     """Rows of the k best scores, (score desc, row asc)."""
     nrows = scores.numel() // stride
@@ -173,6 +212,19 @@ class SearchPlan:
         if self.mode == nat.VX_MODE_PROB:
             return 3 if self.mix else 2
         return 1
+
+    def correlate(self, tspec: torch.Tensor, q: torch.Tensor) -> torch.Tensor:
+        """Per-channel unnormalised correlation maps irfftn(tspec * rfftn(q))
+        (fft_search_best_dot, VESPER_1_prob_new.py:464-491).  tspec: [C, n, n,
+        n/2+1] complex64 (conjugated target spectra), q: [C, n, n, n] float32."""
+        tspec = tspec.contiguous()
+        q = q.contiguous()
+        c = int(q.shape[0])
+        if tuple(q.shape[1:]) != (self.n,) * 3 or tuple(tspec.shape) != (c, self.n, self.n, self.n // 2 + 1):
+            raise VesperError("correlate: shapes do not match the plan's grid edge")
+        out = torch.empty_like(q)
+        nat.call("vx_correlate", self._h, _p(tspec), c, _p(q), _p(out), _stream())
+        return out
 
     def score_rows(self, rot: torch.Tensor, best: torch.Tensor | None = None,
                    trans: torch.Tensor | None = None):
