@@ -90,11 +90,14 @@ int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** out);
 int vx_plan_destroy(vx_plan* plan);
 
 /*
- * Kernel schedule of a plan (the autotuner's knobs; results do not depend on
- * them).  VX_TUNE_XPASS_VARIANT: 0 = x pass accumulating channels in
- * registers, 1 = in LDS.  VX_TUNE_XPASS_TILE: columns per x-pass tile
- * (0 = default).  Grids whose (y, z) slab exceeds LDS always use the pencil
- * passes.
+ * Kernel schedule of a plan (the autotuner's knobs).  VX_TUNE_XPASS_VARIANT:
+ * 0 = FFT x pass accumulating channels in registers, 1 = in LDS (both give
+ * identical results), 2 = split-bf16 MFMA DFT x pass (one correlation group,
+ * N <= 192; scores within 1e-4 relative of the FFT pass).
+ * VX_TUNE_XPASS_TILE: columns per x-pass tile (0 = default; 32 or 64 for
+ * variant 2).  VX_TUNE_SLAB_THREADS: threads per z/y slab workgroup (0 =
+ * default, 512 or 1024 where the slab fits that shape; identical results).
+ * Grids whose (y, z) slab exceeds LDS always use the pencil passes.
  */
 #define VX_TUNE_XPASS_VARIANT 1
 #define VX_TUNE_XPASS_TILE 2
@@ -104,6 +107,7 @@ int vx_plan_destroy(vx_plan* plan);
                                    vx_set_search fails if a query's support does not fit it;
                                    0 = per query.  Set VX_TUNE_PIN_BOX_LO first. */
 #define VX_TUNE_PIN_BOX_LO 5
+#define VX_TUNE_SLAB_THREADS 6
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

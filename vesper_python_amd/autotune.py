@@ -32,7 +32,8 @@ def candidates(n: int, groups: int = 1):
     for t in (2, 4, 8, 12, 16):
         out.append(dict(xpass_variant=1, xpass_tile=t))
     if groups == 1 and n <= 192:
-        out.append(dict(xpass_variant=2, xpass_tile=0))
+        for t in (32, 64):
+            out.append(dict(xpass_variant=2, xpass_tile=t))
     return out
 
 
@@ -96,6 +97,26 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
             print(f"[tune] {cand} total={total:.3f} ms xcorr={ms['xcorr']:.3f}", file=sys.stderr, flush=True)
     results.sort(key=lambda r: r[0])
     total, best, ms = results[0]
+    # slab workgroup shape, on top of the best x pass (the stages are independent)
+    for nt in (1024,):
+        cand = dict(best, slab_threads=nt)
+        try:
+            plan.tune(**cand)
+            plan.score_rows(sample[:plan.batch])
+            ms2 = plan.profile_rows(sample)
+        except Exception as e:
+            if verbose:
+                print("[tune] skip", cand, e, file=sys.stderr)
+            continue
+        t2 = sum(ms2.values())
+        if verbose:
+            print(f"[tune] {cand} total={t2:.3f} ms fwd_zy={ms2['fwd_zy']:.3f} inv_yz={ms2['inv_yz']:.3f}",
+                  file=sys.stderr, flush=True)
+        results.append((t2, cand, ms2))
+        if t2 < total:
+            total, best, ms = t2, cand, ms2
+    if "slab_threads" not in best:
+        best = dict(best, slab_threads=0)
     plan.tune(**best)
     rec = {"schedule": best, "ms_per_row": total / nrows, "rows": nrows,
            "tried": [{"schedule": c, "ms_per_row": t / nrows} for t, c, _ in results]}

@@ -15,9 +15,16 @@
 // in place in one LDS buffer.  Forward transforms only; the inverse
 // (unnormalised, FFTW_BACKWARD) is conj(FFT(conj(x))), applied by callers at
 // their loads/stores.
+//
+// The transform arithmetic below may contract a*b + c into one fma (it only
+// has to meet the search's 1e-4 score tolerance, and it stays deterministic);
+// the file restores the translation unit's no-contraction default at its end,
+// which the parity-critical gather and mixing code relies on.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#pragma clang fp contract(fast)
 
 namespace vx {
 
@@ -307,3 +314,5 @@ __device__ __forceinline__ float2* fft_lds_pp(float2* a, float2* b, const FftDes
 }
 
 }  // namespace vx
+
+#pragma clang fp contract(off)

@@ -650,21 +650,22 @@ k_xcorr_lds(const float2* __restrict__ spec, const float2* __restrict__ tgt, flo
 // x pass on MFMA: the x-axis DFTs as real-form complex matrix products with
 // split bf16 operands (v = hi + lo, hi = bf16(v), lo = bf16(v - hi)); each
 // product is hi*hi + hi*lo + lo*hi accumulated in fp32 (the dropped lo*lo
-// term is ~2^-16 of the product).  Per (tile of 64 columns, rotation):
+// term is ~2^-16 of the product).  Per (tile of 32 or 64 columns, rotation):
 //   Y_c = F X_c          F [2NP x 2KP]: rows (Re, Im) of kx, columns
 //                         (Re, Im) of the query slabs x in [lo, lo + KP)
 //   P   = sum_c Y_c * tgt_c             (fp32 VALU; tgt holds conj spectra)
 //   Z   = G P            G [2NP x 2NP]: unnormalised inverse DFT
-// Wave v owns the row tiles i = v, v + 4, ... (Re rows, and their Im rows
-// i + NPT) for all 64 columns, so no wave repeats another's A operand.  The
-// matrices are stored pre-swizzled in MFMA fragment order (one 512-byte run
-// per 16x16 tile, read straight from L2); the data operand is staged in LDS
-// as [column][k] split bf16.  v_mfma_f32_16x16x16_bf16 lane layout: lane l
-// holds A[l % 16][4 (l / 16) + v], B[4 (l / 16) + v][l % 16] and
-// D[4 (l / 16) + v][l % 16].
+// Wave v of NW owns the row tiles i = v, v + NW, ... (Re rows, and their Im
+// rows i + NPT) for all columns, so no wave repeats another's A operand.  The
+// matrices are stored pre-swizzled in MFMA fragment order (one 1 KB run per
+// 16x32 tile, 16 bytes per lane, read straight from L2); the data operand is
+// staged in LDS as [column][k] split bf16 with a 16-byte row pad.
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l % 16][k], B[k][l % 16] for its
+// eight k (the same eight in both operands, so any fixed assignment of k to
+// (l / 16, v) gives the product; here k = 8 (l / 16) + v), and
+// D[4 (l / 16) + v][l % 16], v < 4.
 // ---------------------------------------------------------------------------
 
-typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
@@ -677,75 +678,85 @@ __device__ __forceinline__ void split_bf16(float v, unsigned short& hi, unsigned
     lo = bf16_rne(v - __uint_as_float((uint32_t)hi << 16));
 }
 
-constexpr int kMfmaCols = 64;      // columns of the y-z plane per workgroup
-constexpr int kMfmaPad = 8;        // LDS row padding (bf16 elements)
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+constexpr int kMfmaPad = 8;        // LDS row padding (bf16 elements, 16 bytes)
 constexpr int kMfmaMaxNPT = 12;    // row tiles of 16: grid edges up to 192
 
-// acc[p][h][j] += A(row tile i + h NPT, k step ks) * B(column tile j), split bf16
-template <int MAXP, int NPT>
-__device__ __forceinline__ void mfma_split_step(floatx4_t (&acc)[MAXP][2][4], const unsigned short* __restrict__ Ah,
+__device__ __forceinline__ bf16x8_t ld_bf16x8(const unsigned short* p) {
+    return __builtin_bit_cast(bf16x8_t, *(const u32x4_t*)p);
+}
+
+// acc[p][h][j] += A(row tile i + h NPT, k step ks) * B(column tile j) with
+// split operands, on v_mfma_f32_16x16x32_bf16 (k = 8 (lane / 16) + v in both
+// operands).  A: fragment-ordered matrix, one 1 KB run per (row tile, k step);
+// B: LDS [column][k] rows of `stride` elements.
+template <int MAXP, int NPT, int CT, int NW>
+__device__ __forceinline__ void mfma_split_step(floatx4_t (&acc)[MAXP][2][CT], const unsigned short* __restrict__ Ah,
                                                 const unsigned short* __restrict__ Al, int KS, int ks,
                                                 const unsigned short* bhp, const unsigned short* blp, int stride,
                                                 int wave, int lane) {
-    const int l16 = lane & 15, l4 = (lane >> 4) * 4;
-    bf16x4_t bh[4], bl[4];
+    const int l16 = lane & 15, l8 = (lane >> 4) * 8;
+    bf16x8_t bh[CT], bl[CT];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        bh[j] = *(const bf16x4_t*)&bhp[(16 * j + l16) * stride + 16 * ks + l4];
-        bl[j] = *(const bf16x4_t*)&blp[(16 * j + l16) * stride + 16 * ks + l4];
+    for (int j = 0; j < CT; ++j) {
+        bh[j] = ld_bf16x8(&bhp[(16 * j + l16) * stride + 32 * ks + l8]);
+        bl[j] = ld_bf16x8(&blp[(16 * j + l16) * stride + 32 * ks + l8]);
     }
 #pragma unroll
     for (int p = 0; p < MAXP; ++p) {
-        const int i = wave + 4 * p;
+        const int i = wave + NW * p;
         if (i >= NPT) break;
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
-            const size_t ro = ((size_t)((i + h * NPT) * KS + ks) * 64 + lane) * 4;
-            const bf16x4_t ah = *(const bf16x4_t*)&Ah[ro];
-            const bf16x4_t al = *(const bf16x4_t*)&Al[ro];
+            const size_t ro = ((size_t)((i + h * NPT) * KS + ks) * 64 + lane) * 8;
+            const bf16x8_t ah = ld_bf16x8(&Ah[ro]);
+            const bf16x8_t al = ld_bf16x8(&Al[ro]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bh[j], acc[p][h][j], 0, 0, 0);
-                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ah, bl[j], acc[p][h][j], 0, 0, 0);
-                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(al, bh[j], acc[p][h][j], 0, 0, 0);
+            for (int j = 0; j < CT; ++j) {
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh[j], acc[p][h][j], 0, 0, 0);
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl[j], acc[p][h][j], 0, 0, 0);
+                acc[p][h][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh[j], acc[p][h][j], 0, 0, 0);
             }
         }
     }
 }
 
-template <int NPT>
-__global__ void __launch_bounds__(256)
+template <int NPT, int CT, int NW>
+__global__ void __launch_bounds__(64 * NW)
 k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
              const unsigned short* __restrict__ Fh, const unsigned short* __restrict__ Fl,
              const unsigned short* __restrict__ Gh, const unsigned short* __restrict__ Gl,
              int n, int nhz, int KP, int lo, int w, int C, int nb, int ntiles) {
+    constexpr int COLS = 16 * CT;
     constexpr int NP = 16 * NPT;
-    constexpr int MAXP = (NPT + 3) / 4;
     constexpr int NP2 = 2 * NP;
+    constexpr int MAXP = (NPT + NW - 1) / NW;
+    constexpr int NT = 64 * NW;
     extern __shared__ unsigned short sm[];
-    const int id = blockIdx.x;
     int b, tile;
-    if (!xcd_tile(id, nb, ntiles, &tile, &b)) return;
+    if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
     const int plane = n * nhz;
-    const int j0 = tile * kMfmaCols;
+    const int j0 = tile * COLS;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int l16 = lane & 15, l4 = (lane >> 4) * 4;
-    const int K2 = 2 * KP, KS = K2 / 16;
+    const int K2 = 2 * KP, KS = K2 / 32;
     const int SK = K2 + kMfmaPad;
     unsigned short* xh = sm;
-    unsigned short* xl = sm + kMfmaCols * SK;
-    float P[MAXP][4][4][2];
+    unsigned short* xl = sm + COLS * SK;
+    float P[MAXP][CT][4][2];
 #pragma unroll
     for (int p = 0; p < MAXP; ++p)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CT; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) { P[p][j][v][0] = 0.f; P[p][j][v][1] = 0.f; }
     for (int c = 0; c < C; ++c) {
         // stage X_c^T: column t, k < KP -> Re of slab lo + k, k >= KP -> Im
         const float2* src = spec + ((size_t)b * C + c) * n * (size_t)plane + j0;
-        for (int e = threadIdx.x; e < kMfmaCols * KP; e += 256) {
-            const int k = e >> 6, t = e & 63;
+        for (int e = threadIdx.x; e < COLS * KP; e += NT) {
+            const int k = e / COLS, t = e - k * COLS;
             float2 v = make_float2(0.f, 0.f);
             if (k < w && j0 + t < plane) v = src[(size_t)(lo + k) * plane + t];
             unsigned short h, l;
@@ -755,23 +766,23 @@ k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, fl
             xh[t * SK + KP + k] = h; xl[t * SK + KP + k] = l;
         }
         __syncthreads();
-        floatx4_t acc[MAXP][2][4];
+        floatx4_t acc[MAXP][2][CT];
 #pragma unroll
         for (int p = 0; p < MAXP; ++p)
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
-                for (int j = 0; j < 4; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+                for (int j = 0; j < CT; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
         for (int ks = 0; ks < KS; ++ks)
-            mfma_split_step<MAXP, NPT>(acc, Fh, Fl, KS, ks, xh, xl, SK, wave, lane);
+            mfma_split_step<MAXP, NPT, CT, NW>(acc, Fh, Fl, KS, ks, xh, xl, SK, wave, lane);
         // P += Y_c * tgt_c
         const float2* tc = tgt + (size_t)c * n * plane + j0;
 #pragma unroll
         for (int p = 0; p < MAXP; ++p) {
-            const int i = wave + 4 * p;
+            const int i = wave + NW * p;
             if (i >= NPT) break;
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < CT; ++j)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int kx = 16 * i + l4 + v, t = 16 * j + l16;
@@ -788,13 +799,13 @@ k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, fl
     // stage P^T as the inverse's data operand: column t, r < NP -> Re kx = r, else Im
     const int SP = NP2 + kMfmaPad;
     unsigned short* ph = sm;
-    unsigned short* pl = sm + kMfmaCols * SP;
+    unsigned short* pl = sm + COLS * SP;
 #pragma unroll
     for (int p = 0; p < MAXP; ++p) {
-        const int i = wave + 4 * p;
+        const int i = wave + NW * p;
         if (i >= NPT) break;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CT; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int kx = 16 * i + l4 + v, t = 16 * j + l16;
@@ -806,22 +817,22 @@ k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, fl
             }
     }
     __syncthreads();
-    floatx4_t acc[MAXP][2][4];
+    floatx4_t acc[MAXP][2][CT];
 #pragma unroll
     for (int p = 0; p < MAXP; ++p)
 #pragma unroll
         for (int h = 0; h < 2; ++h)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
-    for (int ks = 0; ks < NP2 / 16; ++ks)
-        mfma_split_step<MAXP, NPT>(acc, Gh, Gl, NP2 / 16, ks, ph, pl, SP, wave, lane);
+            for (int j = 0; j < CT; ++j) acc[p][h][j] = floatx4_t{0.f, 0.f, 0.f, 0.f};
+    for (int ks = 0; ks < NP2 / 32; ++ks)
+        mfma_split_step<MAXP, NPT, CT, NW>(acc, Gh, Gl, NP2 / 32, ks, ph, pl, SP, wave, lane);
     float2* dst = out + (size_t)b * n * (size_t)plane + j0;
 #pragma unroll
     for (int p = 0; p < MAXP; ++p) {
-        const int i = wave + 4 * p;
+        const int i = wave + NW * p;
         if (i >= NPT) break;
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < CT; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int x = 16 * i + l4 + v, t = 16 * j + l16;
@@ -832,12 +843,12 @@ k_xcorr_mfma(const float2* __restrict__ spec, const float2* __restrict__ tgt, fl
 }
 
 // Fragment-ordered split bf16 DFT matrices of the MFMA x pass (host side).
-// rows 2 NP, columns 2 KP; element (r, k) of tile (r / 16, k / 16) at
-// ((tile_r * KS + tile_k) * 64 + lane) * 4 + v, lane = r % 16 + 16 ((k % 16) / 4),
-// v = k % 4.  fwd: F (columns are query slabs x = lo + k); else G.
+// rows 2 NP, columns K2 (= 2 KP forward, 2 NP inverse); element (r, k) at
+// ((r / 16 * KS + k / 32) * 64 + lane) * 8 + k % 8, lane = r % 16 + 16 ((k % 32) / 8),
+// KS = K2 / 32.  fwd: F (columns are query slabs x = lo + k); else G.
 static void build_dft_matrix(int n, int NP, int KP, int lo, int w, bool fwd,
                              std::vector<unsigned short>& hi, std::vector<unsigned short>& lov) {
-    const int R2 = 2 * NP, K2 = fwd ? 2 * KP : 2 * NP, KS = K2 / 16, half = K2 / 2;
+    const int R2 = 2 * NP, K2 = fwd ? 2 * KP : 2 * NP, KS = K2 / 32, half = K2 / 2;
     hi.assign((size_t)R2 * K2, 0);
     lov.assign((size_t)R2 * K2, 0);
     auto to_bf16 = [](float f) {
@@ -867,45 +878,67 @@ static void build_dft_matrix(int n, int NP, int KP, int lo, int w, bool fwd,
             float hf;
             memcpy(&hf, &hu, 4);
             const unsigned short l = to_bf16(f - hf);
-            const int lane = (r % 16) + 16 * ((k % 16) / 4);
-            const size_t idx = ((size_t)((r / 16) * KS + k / 16) * 64 + lane) * 4 + (k % 4);
+            const int lane = (r % 16) + 16 * ((k % 32) / 8);
+            const size_t idx = ((size_t)((r / 16) * KS + k / 32) * 64 + lane) * 8 + (k % 8);
             hi[idx] = h;
             lov[idx] = l;
         }
     }
 }
 
-static size_t xcorr_mfma_lds(int NP, int KP) {
-    const int s = std::max(2 * KP, 2 * NP) + kMfmaPad;
-    return (size_t)kMfmaCols * s * 2 * sizeof(unsigned short);
+// x-pass MFMA shapes: tile = 32 columns (4 waves) or 64 columns (8 waves)
+static void mfma_shape(int tile, int* ct, int* nw) {
+    if (tile == 32) { *ct = 2; *nw = 4; }
+    else            { *ct = 4; *nw = 8; }
 }
 
-template <int NPT>
+static size_t xcorr_mfma_lds(int NP, int KP, int tile) {
+    int ct, nw;
+    mfma_shape(tile, &ct, &nw);
+    const int s = std::max(2 * KP, 2 * NP) + kMfmaPad;
+    return (size_t)16 * ct * s * 2 * sizeof(unsigned short);
+}
+
+template <int NPT, int CT, int NW>
 static void launch_xcorr_mfma_t(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                                 float2* out, const unsigned short* const* mats, int n, int nhz, int KP,
                                 int lo, int w, int C, int nb, int ntiles) {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_xcorr_mfma<NPT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_xcorr_mfma<NPT>), grid, dim3(256), lds, st, spec, tgt, out, mats[0], mats[1], mats[2],
-                       mats[3], n, nhz, KP, lo, w, C, nb, ntiles);
+        (void)hipFuncSetAttribute((const void*)k_xcorr_mfma<NPT, CT, NW>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+    hipLaunchKernelGGL((k_xcorr_mfma<NPT, CT, NW>), grid, dim3(64 * NW), lds, st, spec, tgt, out, mats[0], mats[1],
+                       mats[2], mats[3], n, nhz, KP, lo, w, C, nb, ntiles);
 }
 
 int launch_xcorr_mfma(const float2* spec, const float2* tgt, float2* out, const unsigned short* const* mats,
-                      int n, int nb, int C, int KP, int lo, int w, hipStream_t st) {
+                      int n, int nb, int C, int KP, int lo, int w, int tile, hipStream_t st) {
     const int nhz = n / 2 + 1;
     const int NPT = (n + 15) / 16;
     VX_REQUIRE(NPT <= kMfmaMaxNPT, VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", n);
-    VX_REQUIRE(KP % 8 == 0 && KP >= w && KP <= 16 * NPT, VX_ERR_STATE, "bad MFMA x-pass width");
-    const size_t lds = xcorr_mfma_lds(16 * NPT, KP);
+    VX_REQUIRE(KP % 16 == 0 && KP >= w && KP <= 16 * NPT, VX_ERR_STATE, "bad MFMA x-pass width");
+    int ct, nw;
+    mfma_shape(tile, &ct, &nw);
+    const size_t lds = xcorr_mfma_lds(16 * NPT, KP, tile);
     VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", n);
-    const int ntiles = (n * nhz + kMfmaCols - 1) / kMfmaCols;
+    const int cols = 16 * ct;
+    const int ntiles = (n * nhz + cols - 1) / cols;
     dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
-    switch (NPT) {
-#define VX_NPT(k) case k: launch_xcorr_mfma_t<k>(grid, lds, st, spec, tgt, out, mats, n, nhz, KP, lo, w, C, nb, ntiles); break;
-        VX_NPT(1) VX_NPT(2) VX_NPT(3) VX_NPT(4) VX_NPT(5) VX_NPT(6)
-        VX_NPT(7) VX_NPT(8) VX_NPT(9) VX_NPT(10) VX_NPT(11) VX_NPT(12)
+    if (ct == 2) {
+        switch (NPT) {
+#define VX_NPT(k) case k: launch_xcorr_mfma_t<k, 2, 4>(grid, lds, st, spec, tgt, out, mats, n, nhz, KP, lo, w, C, nb, ntiles); break;
+            VX_NPT(1) VX_NPT(2) VX_NPT(3) VX_NPT(4) VX_NPT(5) VX_NPT(6)
+            VX_NPT(7) VX_NPT(8) VX_NPT(9) VX_NPT(10) VX_NPT(11) VX_NPT(12)
 #undef VX_NPT
-        default: break;
+            default: break;
+        }
+    } else {
+        switch (NPT) {
+#define VX_NPT(k) case k: launch_xcorr_mfma_t<k, 4, 8>(grid, lds, st, spec, tgt, out, mats, n, nhz, KP, lo, w, C, nb, ntiles); break;
+            VX_NPT(1) VX_NPT(2) VX_NPT(3) VX_NPT(4) VX_NPT(5) VX_NPT(6)
+            VX_NPT(7) VX_NPT(8) VX_NPT(9) VX_NPT(10) VX_NPT(11) VX_NPT(12)
+#undef VX_NPT
+            default: break;
+        }
     }
     VX_LAUNCH_CHECK();
     return VX_OK;
@@ -1247,7 +1280,17 @@ static int allow_lds(K kernel, size_t bytes) {
 // phase fits: z pass (npair x n), unpack (npair x nhz pairs, MAXV/2 each),
 // y pass (nhz x n), inverse pack (npair x n).  Variants (MAXV, threads):
 // (8, 512), (16, 512), (32, 512), (24, 1024).
-static void slab_shape(int n, int nhz, int* nthr, int* maxv) {
+static bool slab_fits(int n, int nhz, int mv, int t) {
+    const int npair = (n + 1) / 2;
+    return n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2);
+}
+
+// Slab workgroup shape: values per thread (MAXV) and threads.  nt_pref 1024
+// asks for the 16 x 1024 shape (code 17) where it fits.
+static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
+    if (nt_pref == 1024 && slab_fits(n, nhz, 16, 1024) && !slab_fits(n, nhz, 8, 512)) {
+        *nthr = 1024; *maxv = 17; return;
+    }
     const int npair = (n + 1) / 2;
     const int opts[4][2] = {{8, 512}, {16, 512}, {32, 512}, {24, 1024}};
     for (auto& o : opts) {
@@ -1266,14 +1309,15 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv) {
     } while (0)
 
 int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
-                  hipStream_t st) {
+                  hipStream_t st, int nt_pref = 0) {
     const int nhz = n / 2 + 1;
     int nthr, maxv;
-    slab_shape(n, nhz, &nthr, &maxv);
+    slab_shape(n, nhz, &nthr, &maxv, nt_pref);
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
     else                 VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz, lo, w);
@@ -1282,14 +1326,15 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
 }
 
 int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
-                  unsigned long long* keys, int nkeys, float* real_out, hipStream_t st) {
+                  unsigned long long* keys, int nkeys, float* real_out, hipStream_t st, int nt_pref = 0) {
     const int nhz = n / 2 + 1;
     int nthr, maxv;
-    slab_shape(n, nhz, &nthr, &maxv);
+    slab_shape(n, nhz, &nthr, &maxv, nt_pref);
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else                 VX_SLAB_LAUNCH(k_inv_yz, 24, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -1408,6 +1453,7 @@ struct vx_plan {
     int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
     int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
     int pin_lo;
+    int slab_nt;      // preferred threads per slab workgroup (0 = default)
     unsigned short* dft;     // MFMA x pass: split bf16 F (hi, lo) then G (hi, lo), fragment order
     size_t dft_elems[2];     // elements of one F matrix, one G matrix
     int dft_lo, dft_w, dft_kp; // box the F matrices were built for (dft_w = 0: none)
@@ -1500,7 +1546,7 @@ extern "C" int vx_plan_info(const vx_plan* p, int64_t* info) {
 static int ensure_dft(vx_plan* p) {
     if (p->xvariant != 2 || (p->dft && p->dft_lo == p->lo && p->dft_w == p->w)) return VX_OK;
     const int n = p->n, NP = 16 * ((n + 15) / 16);
-    const int KP = (p->w + 7) / 8 * 8;
+    const int KP = (p->w + 15) / 16 * 16;
     std::vector<unsigned short> fh, fl, gh, gl;
     build_dft_matrix(n, NP, KP, p->lo, p->w, true, fh, fl);
     build_dft_matrix(n, NP, NP, 0, n, false, gh, gl);
@@ -1644,7 +1690,7 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
             if (value == 2) {
                 VX_REQUIRE(p->G == 1, VX_ERR_UNSUPPORTED, "the MFMA x pass has one correlation group");
                 VX_REQUIRE((p->n + 15) / 16 <= kMfmaMaxNPT &&
-                           xcorr_mfma_lds(16 * ((p->n + 15) / 16), 16 * ((p->n + 15) / 16)) <= kMaxLds,
+                           xcorr_mfma_lds(16 * ((p->n + 15) / 16), 16 * ((p->n + 15) / 16), 64) <= kMaxLds,
                            VX_ERR_UNSUPPORTED, "grid edge %d too large for the MFMA x pass", p->n);
             }
             p->xvariant = value; break;
@@ -1659,6 +1705,11 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
             VX_REQUIRE(value >= 0 && p->pin_lo + value <= p->n, VX_ERR_ARG, "pinned box exceeds the grid");
             p->pin_w = value;
             if (value > 0) { p->lo = p->pin_lo; p->w = value; }
+            break;
+        case VX_TUNE_SLAB_THREADS:
+            VX_REQUIRE(value == 0 || value == 512 || value == 1024, VX_ERR_ARG,
+                       "slab threads must be 0, 512 or 1024");
+            p->slab_nt = value;
             break;
         case VX_TUNE_FORCE_PENCIL:
             if (value) p->pencil = 1;
@@ -1718,7 +1769,7 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
         }
         VX_MARK(1);
         rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st)
-                       : launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st);
+                       : launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt);
         if (rc != VX_OK) return rc;
         VX_MARK(2);
         if (p->xvariant == 2) {
@@ -1726,7 +1777,7 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
                        "MFMA x-pass matrices do not match the support box");
             const unsigned short* mats[4] = {p->dft, p->dft + p->dft_elems[0], p->dft + 2 * p->dft_elems[0],
                                              p->dft + 2 * p->dft_elems[0] + p->dft_elems[1]};
-            rc = launch_xcorr_mfma(p->spec, p->tgt, p->sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, st);
+            rc = launch_xcorr_mfma(p->spec, p->tgt, p->sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, p->xT, st);
         } else {
             rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
                               p->xvariant, p->xT);
@@ -1735,7 +1786,7 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
         VX_MARK(3);
         float* ro = (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr;
         rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st)
-                       : launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st);
+                       : launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st, p->slab_nt);
         if (rc != VX_OK) return rc;
         VX_MARK(4);
         if (p->mode == VX_MODE_PROB && p->mix) {

@@ -178,7 +178,7 @@ class SearchPlan:
                     box_width=int(out[7]))
 
     def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None,
-             pencil: bool | None = None):
+             pencil: bool | None = None, slab_threads: int | None = None):
         """Select kernel schedules.  x-pass variants 0 and 1 (FFT) and the
         pencil switch give identical results; variant 2 (split-bf16 MFMA DFT)
         agrees with them within 1e-4 relative."""
@@ -188,6 +188,8 @@ class SearchPlan:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_VARIANT, int(xpass_variant))
         if xpass_tile is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_TILE, int(xpass_tile))
+        if slab_threads is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_SLAB_THREADS, int(slab_threads))
 
     def pin_box(self, lo: int, width: int):
         """Fix the support box to [lo, lo + width) on every axis (width 0 =
