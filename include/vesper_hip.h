@@ -96,7 +96,7 @@ int vx_plan_destroy(vx_plan* plan);
  * N <= 192; scores within 1e-4 relative of the FFT pass).
  * VX_TUNE_XPASS_TILE: columns per x-pass tile (0 = default; 32 or 64 for
  * variant 2).  VX_TUNE_SLAB_THREADS: threads per z/y slab workgroup (0 =
- * default, 512 or 1024 where the slab fits that shape; identical results).
+ * default = 1024 where the slab fits that shape, else 512; identical results).
  * Grids whose (y, z) slab exceeds LDS always use the pencil passes.
  */
 #define VX_TUNE_XPASS_VARIANT 1

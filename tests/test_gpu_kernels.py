@@ -421,3 +421,33 @@ def test_fft_search_score_trans_1d_modes(gpu, mode):
     eb = float(e.max())
     assert abs(b - eb) <= RTOL * abs(eb)
     assert abs(e[t] - eb) <= RTOL * abs(eb)
+
+
+def test_slab_shapes_identical_at_n175(gpu):
+    """N = 175 (radices 5 and 7): the 1024-thread slab kernels compiled for
+    those radices and the generic 512-thread ones give bit-identical scores,
+    and the rows match the FP64 oracle."""
+    n = 175
+    rng = np.random.default_rng(1750)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    inner = np.linalg.norm(g, axis=-1) <= 30
+    td, tv = _maps(rng, n, 0.02)
+    sd, sv = _maps(rng, n, 0.3)
+    sd[~inner] = 0
+    sv[~inner] = 0
+    angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 170]], np.float64)
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    res = []
+    for nt in (0, 512):
+        _say("slab shape n=175", nt)
+        plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=2)
+        plan.tune(slab_threads=nt)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    o32, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref32")
+    o64, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref64")
+    _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=175")

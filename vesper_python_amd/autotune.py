@@ -98,7 +98,7 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
     results.sort(key=lambda r: r[0])
     total, best, ms = results[0]
     # slab workgroup shape, on top of the best x pass (the stages are independent)
-    for nt in (1024,):
+    for nt in (512,):
         cand = dict(best, slab_threads=nt)
         try:
             plan.tune(**cand)

@@ -43,6 +43,7 @@ struct FftDesc {
     int n;
     int nst;
     int ntw;                     // twiddle table length (float2)
+    unsigned radix_mask;         // bit r set when a radix-r stage occurs
     const FftStage* st;          // device [nst]
     const float2* twid;          // twiddle table: device, or LDS once staged
 };
@@ -229,7 +230,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
 }
 
 // Forward FFT of `nlines` lines in LDS (caller has synchronised after filling).
-template <int MAXV>
+template <int MAXV, unsigned RM = 0xffffffffu>
 __device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int ls, bool line_fast) {
     for (int s = 0; s < d.nst; ++s) {
         const FftStage sg = d.st[s];
@@ -237,14 +238,10 @@ __device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int l
         const int p = sg.P;
         const float pi = sg.Pinv;
         switch (sg.R) {
-            case 2:  fft_stage<2, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 3:  fft_stage<3, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 4:  fft_stage<4, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 5:  fft_stage<5, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 7:  fft_stage<7, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 8:  fft_stage<8, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 11: fft_stage<11, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 13: fft_stage<13, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+#define VX_IP_CASE(r) case r: if constexpr ((RM & (1u << r)) != 0) fft_stage<r, MAXV>(buf, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            VX_IP_CASE(2) VX_IP_CASE(3) VX_IP_CASE(4) VX_IP_CASE(5)
+            VX_IP_CASE(7) VX_IP_CASE(8) VX_IP_CASE(11) VX_IP_CASE(13)
+#undef VX_IP_CASE
             default: break;
         }
     }
@@ -290,6 +287,14 @@ __device__ __forceinline__ void fft_stage_pp(const float2* __restrict__ in, floa
     __syncthreads();
 }
 
+// Radix sets: bit r of a mask enables the radix-r stage code in a kernel
+// (kernels instantiated for the radices of their plan's length carry less
+// code and scalar state); kAllRadices compiles all of them.
+constexpr unsigned radix_bit(int r) { return 1u << r; }
+constexpr unsigned kAllRadices = radix_bit(2) | radix_bit(3) | radix_bit(4) | radix_bit(5) | radix_bit(7) |
+                                 radix_bit(8) | radix_bit(11) | radix_bit(13);
+
+template <unsigned RM = kAllRadices>
 __device__ __forceinline__ float2* fft_lds_pp(float2* a, float2* b, const FftDesc& d, int nlines, int es, int ls,
                                               bool line_fast) {
     for (int s = 0; s < d.nst; ++s) {
@@ -298,14 +303,10 @@ __device__ __forceinline__ float2* fft_lds_pp(float2* a, float2* b, const FftDes
         const int p = sg.P;
         const float pi = sg.Pinv;
         switch (sg.R) {
-            case 2:  fft_stage_pp<2>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 3:  fft_stage_pp<3>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 4:  fft_stage_pp<4>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 5:  fft_stage_pp<5>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 7:  fft_stage_pp<7>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 8:  fft_stage_pp<8>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 11: fft_stage_pp<11>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
-            case 13: fft_stage_pp<13>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+#define VX_PP_CASE(r) case r: if constexpr ((RM & radix_bit(r)) != 0) fft_stage_pp<r>(a, b, d.n, nlines, es, ls, line_fast, p, pi, tw); break;
+            VX_PP_CASE(2) VX_PP_CASE(3) VX_PP_CASE(4) VX_PP_CASE(5)
+            VX_PP_CASE(7) VX_PP_CASE(8) VX_PP_CASE(11) VX_PP_CASE(13)
+#undef VX_PP_CASE
             default: break;
         }
         float2* t = a; a = b; b = t;

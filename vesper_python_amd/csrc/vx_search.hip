@@ -63,6 +63,7 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
         stages[s].P = p;
         stages[s].Pinv = 1.0f / (float)p;
         stages[s].tw = (int)tw.size();
+        d->radix_mask |= radix_bit(R);
         // [j - 1][k]: lanes holding consecutive k read consecutive twiddles
         for (int j = 1; j < R; ++j)
             for (int k = 0; k < p; ++k) {
@@ -306,7 +307,7 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // ---------------------------------------------------------------------------
 
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
-template <int MAXV, int NT>
+template <int MAXV, int NT, unsigned RM = 0xffffffffu>
 __global__ void __launch_bounds__(NT)
 k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w) {
     // slabs x in [lo, lo + w) only; rows y outside [lo, lo + w) are zero, so
@@ -342,7 +343,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
         }
     }
     __syncthreads();
-    fft_lds<MAXV>(lds + l0 * n, fd, l1 - l0, 1, n, false);
+    fft_lds<MAXV, RM>(lds + l0 * n, fd, l1 - l0, 1, n, false);
     // unpack Z = A + iB: A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
     {
         constexpr int MU = (MAXV + 1) / 2;
@@ -376,7 +377,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
         }
         __syncthreads();
     }
-    fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);
+    fft_lds<MAXV, RM>(lds, fd, nhz, nhz, 1, true);
     float2* out = spec + (plane * n + x) * (size_t)n * nhz;
 #pragma unroll
     for (int m = 0; m < MAXV; ++m) {
@@ -641,6 +642,92 @@ k_xcorr_lds(const float2* __restrict__ spec, const float2* __restrict__ tgt, flo
         for (int e = threadIdx.x; e < ne; e += 256) {
             const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
             dst[(size_t)xx * plane + t] = cconj(r[e]);
+        }
+        __syncthreads();
+    }
+}
+
+// LDS-accumulating x pass with software prefetch (tiles of n * T <= 256 * PF
+// values): the next channel's spectra and this channel's target tile are
+// loaded into registers before the FFT starts, so their HBM latency overlaps
+// the transform (barriers wait only for LDS traffic).  Same arithmetic, in
+// the same order, as k_xcorr_lds.
+template <int G, int PF, unsigned RM>
+__global__ void __launch_bounds__(256)
+k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+               FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
+    extern __shared__ float2 lds[];
+    int b, tile;
+    if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
+    float2* A = lds;
+    float2* Bf = lds + n * T;
+    float2* acc = lds + 2 * n * T;                 // [G][n * T]
+    const FftDesc fd = stage_twiddles(fdg, lds + (2 + G) * n * T);
+    const int plane = n * nhz;
+    const int j0 = tile * T;
+    const int nt = min(T, plane - j0);
+    const int ne = n * nt;
+    const float ntinv = 1.0f / (float)nt;
+    // per-thread element offsets in a channel plane (-1: none); bit 30 marks
+    // slabs outside the support box (zero, not loaded)
+    int off[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int e = threadIdx.x + u * 256;
+        off[u] = -1;
+        if (e < ne) {
+            const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+            off[u] = xx * plane + t;
+            if (xx < lo || xx >= lo + w) off[u] |= 1 << 30;
+        }
+    }
+    for (int e = threadIdx.x; e < G * ne; e += 256) acc[e] = make_float2(0.f, 0.f);
+    const float2* sb = spec + (size_t)b * C * n * (size_t)plane + j0;
+    float2 sv[PF], tv[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        sv[u] = make_float2(0.f, 0.f);
+        if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sb[off[u]];
+    }
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+            if (off[u] >= 0) A[threadIdx.x + u * 256] = sv[u];
+        const float2* tc = tgt + (size_t)c * n * plane + j0;
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+            if (off[u] >= 0) tv[u] = tc[off[u] & ~(1 << 30)];
+        if (c + 1 < C) {
+            const float2* sn = sb + (size_t)(c + 1) * n * plane;
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                sv[u] = make_float2(0.f, 0.f);
+                if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
+            }
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        float2* ag = acc + ((grp_mask >> c) & 1) * ne;
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) ag[e] = cadd(ag[e], cmul(r[e], tv[u]));
+        }
+        __syncthreads();
+    }
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) A[e] = cconj(acc[g * ne + e]);
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
         }
         __syncthreads();
     }
@@ -946,7 +1033,7 @@ int launch_xcorr_mfma(const float2* spec, const float2* tgt, float2* out, const 
 
 // Inverse: sum[bg][x] ([ky][kz]) -> real slab ([y][z]) -> argmax key.
 // Optionally writes the real correlation (prob-mode mixing).
-template <int MAXV, int NT>
+template <int MAXV, int NT, unsigned RM = 0xffffffffu>
 __global__ void __launch_bounds__(NT)
 k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
          unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
@@ -972,7 +1059,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
         }
     }
     __syncthreads();
-    fft_lds<MAXV>(lds, fd, nhz, nhz, 1, true);   // conj of the y-inverse
+    fft_lds<MAXV, RM>(lds, fd, nhz, nhz, 1, true);   // conj of the y-inverse
     const int npair = (n + 1) >> 1;
     const int half = n >> 1;
     const bool even = (n & 1) == 0;
@@ -1011,7 +1098,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
             if (dst[m] >= 0) lds[dst[m]] = Zc[m];
         __syncthreads();
     }
-    fft_lds<MAXV>(lds, fd, npair, 1, n, false);
+    fft_lds<MAXV, RM>(lds, fd, npair, 1, n, false);
     // z[n] = conj(F): row 2l = F.x, row 2l+1 = -F.y
     float bv = -INFINITY;
     uint32_t bi = 0xffffffffu;
@@ -1280,15 +1367,21 @@ static int allow_lds(K kernel, size_t bytes) {
 // phase fits: z pass (npair x n), unpack (npair x nhz pairs, MAXV/2 each),
 // y pass (nhz x n), inverse pack (npair x n).  Variants (MAXV, threads):
 // (8, 512), (16, 512), (32, 512), (24, 1024).
+// Lengths whose stages are radix 5 and 7 only (35, 175, 245, ...) get
+// kernels compiled for just those radices: less code, fewer live scalars.
+constexpr unsigned kRadix57 = radix_bit(5) | radix_bit(7);
+
 static bool slab_fits(int n, int nhz, int mv, int t) {
     const int npair = (n + 1) / 2;
     return n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2);
 }
 
-// Slab workgroup shape: values per thread (MAXV) and threads.  nt_pref 1024
-// asks for the 16 x 1024 shape (code 17) where it fits.
+// Slab workgroup shape: values per thread (MAXV) and threads.  nt_pref 0 or
+// 1024 takes the 16 x 1024 shape (code 17) where it fits and the slab needs
+// more than 8 x 512 (faster on MI355X: twice the waves per slab); 512 keeps
+// the 512-thread shapes.
 static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
-    if (nt_pref == 1024 && slab_fits(n, nhz, 16, 1024) && !slab_fits(n, nhz, 8, 512)) {
+    if (nt_pref != 512 && slab_fits(n, nhz, 16, 1024) && !slab_fits(n, nhz, 8, 512)) {
         *nthr = 1024; *maxv = 17; return;
     }
     const int npair = (n + 1) / 2;
@@ -1307,6 +1400,11 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
         allow_lds(KER<MV, NTH>, lds);                                                  \
         hipLaunchKernelGGL((KER<MV, NTH>), grid, dim3(NTH), lds, st, __VA_ARGS__);     \
     } while (0)
+#define VX_SLAB_LAUNCH_RM(KER, MV, NTH, RMV, ...)                                      \
+    do {                                                                               \
+        allow_lds(KER<MV, NTH, RMV>, lds);                                             \
+        hipLaunchKernelGGL((KER<MV, NTH, RMV>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
+    } while (0)
 
 int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
                   hipStream_t st, int nt_pref = 0) {
@@ -1317,6 +1415,8 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 17 && fd.radix_mask == kRadix57)
+        VX_SLAB_LAUNCH_RM(k_fwd_zy, 16, 1024, kRadix57, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
@@ -1334,6 +1434,8 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 17 && fd.radix_mask == kRadix57)
+        VX_SLAB_LAUNCH_RM(k_inv_yz, 16, 1024, kRadix57, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -1351,6 +1453,7 @@ static int xcorr_T(int n) {
 }
 
 constexpr int kXcorrMaxE = 12;   // tile values per thread in the x pass
+constexpr int kXcorrPF = 8;      // prefetched tile values per thread (LDS-accumulating x pass)
 
 static int xcorr_pp_T(int n) {
     // n * T <= 256 threads * kXcorrMaxE values; T <= 16 keeps two tiles of LDS small
@@ -1358,6 +1461,23 @@ static int xcorr_pp_T(int n) {
     if (T > 16) T = 16;
     if (T < 1) T = 1;
     return T;
+}
+
+// Radix set of a plan's length (bit r set when a radix-r stage occurs).
+
+static unsigned radix_mask(const FftDesc& fd) {
+    return fd.radix_mask;
+}
+
+template <int G, unsigned RM>
+static void launch_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt, float2* out,
+                      const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo,
+                      int w) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_xcorr_lds_pf<G, kXcorrPF, RM>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_xcorr_lds_pf<G, kXcorrPF, RM>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb,
+                       C, T, ntiles, grp_mask, lo, w);
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
@@ -1381,6 +1501,18 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
         VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
         const int ntiles = (n * nhz + T - 1) / T;
         dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
+        if (n * T <= 256 * kXcorrPF) {
+            const unsigned rm = radix_mask(fd);
+            if (G == 1) {
+                if (rm == kRadix57) launch_pf<1, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+                else                launch_pf<1, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            } else {
+                if (rm == kRadix57) launch_pf<2, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+                else                launch_pf<2, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            }
+            VX_LAUNCH_CHECK();
+            return VX_OK;
+        }
         if (lds > 65536) {
             if (G == 1) VX_HIP(hipFuncSetAttribute((const void*)k_xcorr_lds<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             else        VX_HIP(hipFuncSetAttribute((const void*)k_xcorr_lds<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
