@@ -144,7 +144,8 @@ def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
     return {"recall_top10": proposer.recall(top, truth), "fine_rows": R, "fft_rows_scored": scored + ncoarse,
             "seconds": round(dt, 3), "rotation_equivalents_per_s": round(rate, 1),
             "speedup_vs_plain": round(rate / plain_rate, 2),
-            "method": "MLP (80-256-256-256-1, bf16/MFMA) trained on the -A 30 landscape, 2 rounds x 2.5 % of rows"}
+            "method": "MLP (80-256-256-256-1, bf16/MFMA) trained on the -A 30 landscape, 2 rounds x 2.5 % of rows, "
+                      "then neighbour sweeps around the best rows"}
 
 
 def main():

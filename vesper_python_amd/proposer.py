@@ -222,15 +222,52 @@ class Proposer:
         return sel[torch.from_numpy(t.astype(np.int64)).to(sel.device)].cpu().numpy(), b, int(sel.numel())
 
 
+def grid_shape(angles):
+    """(nx, nz) when `angles` is a calc_angle_comb grid (row = iz nx^2 + ix nx + iy
+    for angle (x[ix], x[iy], z[iz])), else None."""
+    a = np.asarray(angles, dtype=np.float64)
+    xs = np.unique(a[:, 0])
+    zs = np.unique(a[:, 2])
+    nx, nz = len(xs), len(zs)
+    if nx < 2 or len(a) != nx * nx * nz:
+        return None
+    iz, rem = np.divmod(np.arange(len(a)), nx * nx)
+    ix, iy = np.divmod(rem, nx)
+    if not (np.array_equal(a[:, 0], xs[ix]) and np.array_equal(a[:, 1], xs[iy]) and np.array_equal(a[:, 2], zs[iz])):
+        return None
+    return nx, nz
+
+
+def neighbour_rows(rows, nx, nz):
+    """Grid neighbours (+-1 step per Euler angle; x and y wrap at 360, z is
+    clamped to the grid) of the given rows of a calc_angle_comb grid."""
+    rows = np.asarray(rows, dtype=np.int64)
+    iz, rem = np.divmod(rows, nx * nx)
+    ix, iy = np.divmod(rem, nx)
+    out = []
+    for dz in (-1, 0, 1):
+        z = iz + dz
+        ok = (z >= 0) & (z < nz)
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                r = z * nx * nx + ((ix + dx) % nx) * nx + (iy + dy) % nx
+                out.append(r[ok])
+    return np.unique(np.concatenate(out))
+
+
 def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, topn=10, epochs=40, lr=2e-3,
-                   seed=0):
+                   seed=0, sweeps=3):
     """Coarse landscape -> train -> propose a share of the rows -> FFT-score
     them -> retrain on everything scored so far -> propose the next share of
-    the rows not yet scored ... (`rounds` shares of frac / rounds each).
+    the rows not yet scored ... (`rounds` shares of frac / rounds each); then
+    up to `sweeps` passes that FFT-score the unscored grid neighbours of the
+    current top rows (hill climbing on the exact scores).
     Returns (top rows into angles, number of fine rows FFT-scored, coarse rows)."""
     dev = pr.td.device
     crot, cX = pr.rows_features(coarse_angles)
-    cy = zscore(landscape(pr.plan, crot))
+    cs = landscape(pr.plan, crot).double()
+    cmean, cstd = cs.mean(), cs.std() + 1e-30
+    cy = ((cs - cmean) / cstd).float()
     rot, X = pr.rows_features(angles)
     R = int(rot.shape[0])
     scored = torch.zeros(R, dtype=torch.bool, device=dev)
@@ -261,8 +298,17 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
         scored[sel] = True
         trainX.append(X.index_select(0, sel))
         # labels on the coarse landscape's scale
-        cs = landscape(pr.plan, crot).double()
-        trainy.append(((b[:, 0].double() - cs.mean()) / (cs.std() + 1e-30)).float())
+        trainy.append(((b[:, 0].double() - cmean) / cstd).float())
+    shape = grid_shape(angles) if sweeps else None
+    for _ in range(sweeps if shape else 0):
+        top = engine.topn(best_all, topn)
+        nb = torch.from_numpy(neighbour_rows(top, *shape)).to(dev)
+        nb = nb[~scored[nb]]
+        if nb.numel() == 0:
+            break
+        b, _ = pr.plan.score_rows(rot.index_select(0, nb).contiguous())
+        best_all[nb] = b[:, 0]
+        scored[nb] = True
     top = engine.topn(best_all, topn)
     return top, int(scored.sum().item()), int(crot.shape[0])
 
