@@ -244,6 +244,14 @@ def main():
                 "support_box": [info["box_lo"], info["box_width"]],
                 "stage_GBps": {k: round(sb[k] / (stage_ms[k] / launches / 1e3) / 1e9, 1) for k in sb}}
 
+    # whole path per scored row, V mode: source data + vectors (16 N^3) + the
+    # forward outputs and the spectra read by the fused multiply-sum-inverse
+    # (72 Nh), the per-row figure the row rate is priced against
+    nh = n * n * (n // 2 + 1)
+    row_bytes = 16 * n ** 3 + 72 * nh
+    roofline["path"] = {"bytes_per_row": row_bytes, "achieved": round(value * row_bytes / 1e9, 1),
+                        "frac": round(value * row_bytes / 1e9 / PEAK_HBM_GBS, 4),
+                        "model": "16 N^3 + 72 N^2 (N/2+1) bytes per row"}
     if dom == "xcorr":
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
