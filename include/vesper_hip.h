@@ -91,9 +91,10 @@ int vx_plan_destroy(vx_plan* plan);
 
 /*
  * Kernel schedule of a plan (the autotuner's knobs).  VX_TUNE_XPASS_VARIANT:
- * 0 = FFT x pass accumulating channels in registers, 1 = in LDS (both give
- * identical results), 2 = split-bf16 MFMA DFT x pass (one correlation group,
- * N <= 192; scores within 1e-4 relative of the FFT pass).
+ * 0 = FFT x pass accumulating channels in registers, 1 = in LDS, 3 = in
+ * registers with the next channel prefetched (tiles of N x tile <= 2048; 0,
+ * 1 and 3 give identical results), 2 = split-bf16 MFMA DFT x pass (one
+ * correlation group, N <= 192; scores within 1e-4 relative of the FFT pass).
  * VX_TUNE_XPASS_TILE: columns per x-pass tile (0 = default; 32 or 64 for
  * variant 2).  VX_TUNE_SLAB_THREADS: threads per z/y slab workgroup (0 =
  * default = 1024 where the slab fits that shape, else 512; identical results).

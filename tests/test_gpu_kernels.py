@@ -451,3 +451,24 @@ def test_slab_shapes_identical_at_n175(gpu):
     o32, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref32")
     o64, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref64")
     _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=175")
+
+
+@pytest.mark.parametrize("n,mode", [(35, nat.VX_MODE_VEC), (24, nat.VX_MODE_CC)])
+def test_xpass_variants_identical(gpu, n, mode):
+    """The FFT x-pass kernels (register, LDS, register-prefetch; several
+    tiles) give bit-identical scores and translations."""
+    rng = np.random.default_rng(1100 + n)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.3)
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (10, 3)))).cuda()
+    res = []
+    for variant, tile in ((0, 0), (1, 4), (1, 8), (3, 4), (3, 8)):
+        _say("x-pass variant", n, variant, tile)
+        plan = engine.SearchPlan(n, mode, batch=4)
+        plan.tune(xpass_variant=variant, xpass_tile=tile)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), None, 0.3)
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), None, 0.3)
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+    for b, t in res[1:]:
+        assert np.array_equal(b, res[0][0]) and np.array_equal(t, res[0][1])

@@ -733,6 +733,97 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     }
 }
 
+// Register-accumulating form of k_xcorr_lds_pf: each thread owns the same
+// tile elements in every phase, so the group sums live in registers and the
+// workgroup needs two tile buffers of LDS instead of 2 + G (more workgroups
+// per CU).  Same arithmetic, in the same order.
+template <int G, int PF, unsigned RM>
+__global__ void __launch_bounds__(256)
+k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
+               FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
+    extern __shared__ float2 lds[];
+    int b, tile;
+    if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
+    float2* A = lds;
+    float2* Bf = lds + n * T;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
+    const int plane = n * nhz;
+    const int j0 = tile * T;
+    const int nt = min(T, plane - j0);
+    const int ne = n * nt;
+    const float ntinv = 1.0f / (float)nt;
+    int off[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        const int e = threadIdx.x + u * 256;
+        off[u] = -1;
+        if (e < ne) {
+            const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+            off[u] = xx * plane + t;
+            if (xx < lo || xx >= lo + w) off[u] |= 1 << 30;
+        }
+    }
+    float2 acc[G][PF];
+#pragma unroll
+    for (int g = 0; g < G; ++g)
+#pragma unroll
+        for (int u = 0; u < PF; ++u) acc[g][u] = make_float2(0.f, 0.f);
+    const float2* sb = spec + (size_t)b * C * n * (size_t)plane + j0;
+    float2 sv[PF], tv[PF];
+#pragma unroll
+    for (int u = 0; u < PF; ++u) {
+        sv[u] = make_float2(0.f, 0.f);
+        if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sb[off[u]];
+    }
+    for (int c = 0; c < C; ++c) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+            if (off[u] >= 0) A[threadIdx.x + u * 256] = sv[u];
+        const float2* tc = tgt + (size_t)c * n * plane + j0;
+#pragma unroll
+        for (int u = 0; u < PF; ++u)
+            if (off[u] >= 0) tv[u] = tc[off[u] & ~(1 << 30)];
+        if (c + 1 < C) {
+            const float2* sn = sb + (size_t)(c + 1) * n * plane;
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                sv[u] = make_float2(0.f, 0.f);
+                if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
+            }
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        const int g = G == 1 ? 0 : (int)((grp_mask >> c) & 1);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) {
+                const float2 pr = cmul(r[e], tv[u]);
+                if (G == 1 || g == 0) acc[0][u] = cadd(acc[0][u], pr);
+                else                  acc[G - 1][u] = cadd(acc[G - 1][u], pr);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) A[e] = cconj(acc[g][u]);
+        }
+        __syncthreads();
+        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
+#pragma unroll
+        for (int u = 0; u < PF; ++u) {
+            const int e = threadIdx.x + u * 256;
+            if (off[u] >= 0) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
+        }
+        __syncthreads();
+    }
+}
+
 // ---------------------------------------------------------------------------
 // x pass on MFMA: the x-axis DFTs as real-form complex matrix products with
 // split bf16 operands (v = hi + lo, hi = bf16(v), lo = bf16(v - hi)); each
@@ -1480,10 +1571,40 @@ static void launch_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec,
                        C, T, ntiles, grp_mask, lo, w);
 }
 
+template <int G, unsigned RM>
+static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
+                          float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
+                          unsigned grp_mask, int lo, int w) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, kXcorrPF, RM>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_xcorr_reg_pf<G, kXcorrPF, RM>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb,
+                       C, T, ntiles, grp_mask, lo, w);
+}
+
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
                  int nb, int C, int G, unsigned grp_mask, int lo, int w, hipStream_t st,
                  int xvariant = 0, int xT = 0) {
     const int nhz = n / 2 + 1;
+    if (tgt != nullptr && xvariant == 3) {
+        const int T = xT > 0 ? xT : 8;
+        VX_REQUIRE(n * T <= 256 * kXcorrPF, VX_ERR_UNSUPPORTED,
+                   "x-pass variant 3 needs grid edge x tile <= %d", 256 * kXcorrPF);
+        const size_t lds = ((size_t)2 * n * T + fd.ntw) * sizeof(float2);
+        VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
+        const int ntiles = (n * nhz + T - 1) / T;
+        dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
+        const unsigned rm = radix_mask(fd);
+        if (G == 1) {
+            if (rm == kRadix57) launch_reg_pf<1, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            else                launch_reg_pf<1, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+        } else {
+            if (rm == kRadix57) launch_reg_pf<2, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            else                launch_reg_pf<2, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+        }
+        VX_LAUNCH_CHECK();
+        return VX_OK;
+    }
     if (tgt == nullptr) {
         // target preparation: forward x FFT + conj per channel (once per search)
         const int T = xcorr_T(n);
@@ -1580,7 +1701,7 @@ struct vx_plan {
     bool have_target, have_search, mix;
     bool q_dirty;     // q / qd hold non-zero values outside the support box
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
-    int xvariant;     // x-pass kernel: 0 register-accumulating, 1 LDS-accumulating, 2 MFMA DFT
+    int xvariant;     // x-pass kernel: 0 register FFT, 1 LDS FFT, 2 MFMA DFT, 3 register-prefetch FFT
     int xT;           // x-pass tile width (columns), 0 = default
     int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
     int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
@@ -1818,7 +1939,7 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
     VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
     switch (key) {
         case VX_TUNE_XPASS_VARIANT:
-            VX_REQUIRE(value >= 0 && value <= 2, VX_ERR_ARG, "x-pass variant must be 0, 1 or 2");
+            VX_REQUIRE(value >= 0 && value <= 3, VX_ERR_ARG, "x-pass variant must be 0, 1, 2 or 3");
             if (value == 2) {
                 VX_REQUIRE(p->G == 1, VX_ERR_UNSUPPORTED, "the MFMA x pass has one correlation group");
                 VX_REQUIRE((p->n + 15) / 16 <= kMfmaMaxNPT &&
@@ -1962,7 +2083,7 @@ extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float*
     const size_t n3 = (size_t)n * n * n;
     const size_t nh = (size_t)n * n * p->nhz;
     const float2* ts = (const float2*)tspec;
-    const int xv = p->xvariant == 2 ? 1 : p->xvariant;
+    const int xv = p->xvariant == 0 ? 0 : 1;   // the FFT x passes valid for every grid edge
     for (int c = 0; c < nch; ++c) {
         const float* qc = q + (size_t)c * n3;
         int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st)

@@ -179,9 +179,9 @@ class SearchPlan:
 
     def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None,
              pencil: bool | None = None, slab_threads: int | None = None):
-        """Select kernel schedules.  x-pass variants 0 and 1 (FFT) and the
-        pencil switch give identical results; variant 2 (split-bf16 MFMA DFT)
-        agrees with them within 1e-4 relative."""
+        """Select kernel schedules.  x-pass variants 0, 1 and 3 (FFT), the slab
+        shape and the pencil switch give identical results; variant 2
+        (split-bf16 MFMA DFT) agrees with them within 1e-4 relative."""
         if pencil is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_FORCE_PENCIL, int(bool(pencil)))
         if xpass_variant is not None:
