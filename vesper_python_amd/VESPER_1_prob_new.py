@@ -30,8 +30,9 @@ __all__ = ["mrc_obj", "mrc_set_vox_size", "fastVEC", "doFastVEC", "rot_mrc", "ro
            "search_map_fft_prob", "rot_and_search_fft", "rot_and_search_fft_prob", "get_score",
            "show_vec", "calc_angle_comb", "euler2rot", "convert_trans"]
 
-# rotations per launch of the search kernels (overridable by callers)
-SEARCH_BATCH = 8
+# rotations per launch of the search kernels (None: batch.auto_batch of the
+# grid edge; overridable by callers)
+SEARCH_BATCH = None
 
 
 class mrc_obj:
@@ -198,7 +199,8 @@ class _Search:
         self.search = mrc_search
         self.tdata, self.tvec = _dev_maps(mrc_target)
         self.sdata, self.svec = _dev_maps(mrc_search)
-        self.plan = engine.SearchPlan(self.n, _MODE_OF[mode], batch or SEARCH_BATCH)
+        from .batch import auto_batch
+        self.plan = engine.SearchPlan(self.n, _MODE_OF[mode], batch or SEARCH_BATCH or auto_batch(self.n))
         ave = mrc_target.ave if mrc_target.ave is not None else 0.0
         self.plan.set_target(self.tdata, self.tvec, None, ave)
         self.plan.set_search(self.sdata, self.svec, None, ave)

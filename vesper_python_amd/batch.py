@@ -22,8 +22,11 @@ from . import engine
 from .geometry import calc_angle_comb, rotation_rows
 
 
-def auto_batch(n: int) -> int:
-    return int(max(8, min(64, round(8 * (175.0 / n) ** 3))))
+def auto_batch(n: int, mode_channels: int = 3) -> int:
+    """Rotations per launch: 32 at grid edge 175 (measured optimum on
+    MI355X), scaled by the grid volume so the workspace stays near 5 GB;
+    between 8 and 128."""
+    return int(max(8, min(128, round(32 * (175.0 / n) ** 3 * 3.0 / max(1, mode_channels)))))
 
 
 class GraphedSearch:

@@ -36,7 +36,8 @@ class ProbSearch:
         self.sdata, self.svec = _dev_maps(search)
         self.tprob = _dev_prob(tp)
         self.sprob = _dev_prob(sp)
-        self.plan = engine.SearchPlan(self.n, nat.VX_MODE_PROB, batch or SEARCH_BATCH)
+        from .batch import auto_batch
+        self.plan = engine.SearchPlan(self.n, nat.VX_MODE_PROB, batch or SEARCH_BATCH or auto_batch(self.n, 7))
         self.plan.set_target(self.tdata, self.tvec, self.tprob, 0.0)
         self.plan.set_search(self.sdata, self.svec, self.sprob, 0.0)
         self.rd3 = 1.0 / (self.n ** 3)
