@@ -64,7 +64,7 @@ def pmc_traffic(workload: str, batch: int, schedule, box_width: int):
     ``traffic: workload=W batch=B variant=V tile=T box=W bytes=X``; None when
     no line matches this run's workload, batch, x-pass schedule and box."""
     if not isinstance(schedule, dict):
-        schedule = {"xpass_variant": 1, "xpass_tile": 8}
+        schedule = {"xpass_variant": 3, "xpass_tile": 8}
     want = {"workload": workload, "batch": str(batch), "variant": str(schedule.get("xpass_variant")),
             "tile": str(schedule.get("xpass_tile")), "box": str(box_width)}
     try:

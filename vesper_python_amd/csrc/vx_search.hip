@@ -1747,7 +1747,10 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     memset(p, 0, sizeof(*p));
     p->n = n; p->nhz = nhz; p->mode = mode; p->batch = batch; p->flags = flags;
     p->lo = 0; p->w = n; p->q_dirty = true;
-    p->xvariant = 1; p->xT = 0;   // LDS-accumulating x pass, default tile
+    // register-accumulating prefetch x pass with 8-column tiles where it
+    // applies (the autotuner's usual pick on MI355X), else the LDS one
+    p->xvariant = n * 8 <= 256 * kXcorrPF ? 3 : 1;
+    p->xT = 0;
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
