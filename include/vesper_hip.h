@@ -109,6 +109,9 @@ int vx_plan_destroy(vx_plan* plan);
                                    0 = per query.  Set VX_TUNE_PIN_BOX_LO first. */
 #define VX_TUNE_PIN_BOX_LO 5
 #define VX_TUNE_SLAB_THREADS 6
+#define VX_TUNE_STREAMS 7 /* 2: two batches in flight on two workspaces and streams (the
+                             second stream is forked from and joined to the caller's);
+                             1 = one (default).  Identical results. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

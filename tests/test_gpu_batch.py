@@ -78,3 +78,59 @@ def test_uneven_pairs_graph_equals_eager_and_oracle(gpu, tmp_path):
     o = ref.search_map_fft(grid(t), grid(s), TopN=10, ang=30.0, prec="ref32", refine=False)
     assert g[1]["top"].tolist() == list(o["topn"])
     assert np.allclose(g[1]["scores"], o["scores"], rtol=1e-4, atol=1e-6 * np.abs(o["scores"]).max())
+
+
+def test_two_streams_identical_and_capturable(gpu):
+    """Two batches in flight (second workspace on a forked stream): results
+    bit-identical to one stream, eagerly and in graph replays."""
+    n = 26
+    rng = np.random.default_rng(31)
+    td, tv = _maps(rng, n)
+    sd, sv = _maps(rng, n)
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (53, 3)))).cuda()
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=8)
+    plan.set_target(engine.to_dev(td, torch.float64), engine.to_dev(tv, torch.float32))
+    plan.set_search(engine.to_dev(sd, torch.float64), engine.to_dev(sv, torch.float32))
+    print("[case] streams 1", flush=True)
+    b1, t1 = plan.score_rows(rot)
+    b1, t1 = b1.cpu().numpy(), t1.cpu().numpy()
+    plan.tune(streams=2)
+    print("[case] streams 2 eager", flush=True)
+    b2, t2 = plan.score_rows(rot)
+    torch.cuda.synchronize()
+    assert np.array_equal(b2.cpu().numpy(), b1) and np.array_equal(t2.cpu().numpy(), t1)
+    gs = batch.GraphedSearch(plan, rot, 10)
+    for rep in range(3):
+        print("[case] streams 2 replay", rep, flush=True)
+        b, t, _ = gs.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(b.cpu().numpy(), b1) and np.array_equal(t.cpu().numpy(), t1)
+        plan.score_rows(rot[:16])
+        torch.cuda.synchronize()
+
+
+def test_smaller_support_after_larger_clears_planes(gpu):
+    """A second query with a smaller support box on the same plan: the planes
+    written for the first box are cleared (results equal a fresh plan's)."""
+    n = 30
+    rng = np.random.default_rng(41)
+    td, tv = _maps(rng, n)
+    big_d, big_v = _maps(rng, n, 0.4)
+    small_d, small_v = _maps(rng, n, 0.4)
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - n / 2
+    small = np.linalg.norm(g, axis=-1) > n / 6
+    small_d[small] = 0
+    small_v[small] = 0
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (12, 3)))).cuda()
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=4)
+    plan.set_target(engine.to_dev(td, torch.float64), engine.to_dev(tv, torch.float32))
+    plan.set_search(engine.to_dev(big_d, torch.float64), engine.to_dev(big_v, torch.float32))
+    plan.score_rows(rot)
+    plan.set_search(engine.to_dev(small_d, torch.float64), engine.to_dev(small_v, torch.float32))
+    b, t = plan.score_rows(rot)
+    fresh = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=4)
+    fresh.set_target(engine.to_dev(td, torch.float64), engine.to_dev(tv, torch.float32))
+    fresh.set_search(engine.to_dev(small_d, torch.float64), engine.to_dev(small_v, torch.float32))
+    fb, ft = fresh.score_rows(rot)
+    assert plan.info()["box_width"] < n
+    assert np.array_equal(b.cpu().numpy(), fb.cpu().numpy()) and np.array_equal(t.cpu().numpy(), ft.cpu().numpy())

@@ -120,6 +120,31 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
             total, best, ms = t2, cand, ms2
     if "slab_threads" not in best:
         best = dict(best, slab_threads=0)
+    # two batches in flight: only a wall-clock measurement shows the overlap
+    if nrows >= 2 * plan.batch:
+        walls = {}
+        for ns in (1, 2):
+            try:
+                plan.tune(**best, streams=ns)
+                plan.score_rows(sample)
+                torch.cuda.synchronize()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                for _ in range(3):
+                    plan.score_rows(sample)
+                e1.record()
+                torch.cuda.synchronize()
+                walls[ns] = e0.elapsed_time(e1) / 3
+            except Exception as e:
+                if verbose:
+                    print("[tune] skip streams", ns, e, file=sys.stderr)
+        if verbose and walls:
+            print(f"[tune] streams wall ms {walls}", file=sys.stderr, flush=True)
+        if 2 in walls and 1 in walls and walls[2] < walls[1]:
+            best = dict(best, streams=2)
+        else:
+            best = dict(best, streams=1)
     plan.tune(**best)
     rec = {"schedule": best, "ms_per_row": total / nrows, "rows": nrows,
            "tried": [{"schedule": c, "ms_per_row": t / nrows} for t, c, _ in results]}

@@ -1707,6 +1707,15 @@ struct vx_plan {
     int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
     int pin_lo;
     int slab_nt;      // preferred threads per slab workgroup (0 = default)
+    int nstreams;     // 2: alternate batches over two workspaces and streams
+    float* q2;        // second workspace (nstreams == 2)
+    double* qd2;
+    float2* spec2;
+    float2* sum2;
+    unsigned long long* keys2;
+    float* real_out2;
+    hipStream_t st2;
+    hipEvent_t ev_fork, ev_join;
     unsigned short* dft;     // MFMA x pass: split bf16 F (hi, lo) then G (hi, lo), fragment order
     size_t dft_elems[2];     // elements of one F matrix, one G matrix
     int dft_lo, dft_w, dft_kp; // box the F matrices were built for (dft_w = 0: none)
@@ -1717,9 +1726,13 @@ struct vx_plan {
 static void plan_free(vx_plan* p) {
     if (!p) return;
     void* ptrs[] = {p->twid, p->svec, p->sscal, p->sprob, p->sdat, p->q, p->qd, p->spec,
-                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft};
+                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft,
+                    p->q2, p->qd2, p->spec2, p->sum2, p->keys2, p->real_out2};
     for (void* x : ptrs)
         if (x) (void)hipFree(x);
+    if (p->st2) (void)hipStreamDestroy(p->st2);
+    if (p->ev_fork) (void)hipEventDestroy(p->ev_fork);
+    if (p->ev_join) (void)hipEventDestroy(p->ev_join);
     delete p;
 }
 
@@ -1751,6 +1764,7 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     // applies (the autotuner's usual pick on MI355X), else the LDS one
     p->xvariant = n * 8 <= 256 * kXcorrPF ? 3 : 1;
     p->xT = 0;
+    p->nstreams = 1;
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
@@ -1897,6 +1911,8 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
             lo = plo;
             hi = phi;
         }
+        // planes hold the previous box's values outside the new box: clear them
+        if (lo != p->lo || hi - lo + 1 != p->w) p->q_dirty = true;
         p->lo = lo;
         p->w = hi - lo + 1;
     }
@@ -1938,6 +1954,36 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     return VX_OK;
 }
 
+// Second workspace and stream for two batches in flight (VX_TUNE_STREAMS 2).
+static int alloc_second_workspace(vx_plan* p) {
+    const size_t n3 = (size_t)p->n * p->n * p->n;
+    const size_t nh = (size_t)p->n * p->n * p->nhz;
+    const int B = p->batch;
+    struct Req { void** ptr; size_t bytes; };
+    Req reqs[] = {{(void**)&p->q2, (size_t)B * p->C * n3 * sizeof(float)},
+                  {(void**)&p->qd2, p->kind == KIND_DATA64 ? (size_t)B * n3 * sizeof(double) : 0},
+                  {(void**)&p->spec2, (size_t)B * p->C * nh * sizeof(float2)},
+                  {(void**)&p->sum2, (size_t)B * p->G * nh * sizeof(float2)},
+                  {(void**)&p->keys2, (size_t)B * p->nkeys * sizeof(unsigned long long)},
+                  {(void**)&p->real_out2, p->mode == VX_MODE_PROB ? (size_t)B * 2 * n3 * sizeof(float) : 0}};
+    for (auto& r : reqs) {
+        if (r.bytes == 0) continue;
+        if (hipMalloc(r.ptr, r.bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            *r.ptr = nullptr;
+            set_error("device allocation of %zu bytes for the second workspace failed", r.bytes);
+            return VX_ERR_NOMEM;
+        }
+        p->bytes += r.bytes;
+    }
+    VX_HIP(hipMemset(p->q2, 0, (size_t)B * p->C * n3 * sizeof(float)));
+    if (p->qd2) VX_HIP(hipMemset(p->qd2, 0, (size_t)B * n3 * sizeof(double)));
+    VX_HIP(hipStreamCreateWithFlags(&p->st2, hipStreamNonBlocking));
+    VX_HIP(hipEventCreateWithFlags(&p->ev_fork, hipEventDisableTiming));
+    VX_HIP(hipEventCreateWithFlags(&p->ev_join, hipEventDisableTiming));
+    return VX_OK;
+}
+
 extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
     VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
     switch (key) {
@@ -1960,7 +2006,19 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
         case VX_TUNE_PIN_BOX_WIDTH:
             VX_REQUIRE(value >= 0 && p->pin_lo + value <= p->n, VX_ERR_ARG, "pinned box exceeds the grid");
             p->pin_w = value;
-            if (value > 0) { p->lo = p->pin_lo; p->w = value; }
+            if (value > 0 && (p->lo != p->pin_lo || p->w != value)) {
+                p->lo = p->pin_lo;
+                p->w = value;
+                p->q_dirty = true;
+            }
+            break;
+        case VX_TUNE_STREAMS:
+            VX_REQUIRE(value == 1 || value == 2, VX_ERR_ARG, "streams must be 1 or 2");
+            if (value == 2 && !p->q2) {
+                int rc = alloc_second_workspace(p);
+                if (rc != VX_OK) return rc;
+            }
+            p->nstreams = value;
             break;
         case VX_TUNE_SLAB_THREADS:
             VX_REQUIRE(value == 0 || value == 512 || value == 1024, VX_ERR_ARG,
@@ -1996,78 +2054,121 @@ extern "C" int vx_set_prob_stats(vx_plan* p, int enable, double alpha, double vs
 // 3 inv_yz, 4 mixed, 5 finalize.
 constexpr int kStages = 6;
 
-static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
-                           hipStream_t st, hipEvent_t* ev, double* stage_ms) {
-    VX_REQUIRE(p && rot && best && trans, VX_ERR_ARG, "NULL argument");
-    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
-    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+// One batch workspace: query planes, spectra, correlation spectrum, keys.
+struct Workspace {
+    float* q;
+    double* qd;
+    float2* spec;
+    float2* sum;
+    unsigned long long* keys;
+    float* real_out;
+};
+
+// Score rows [r0, r0 + nb) in workspace ws on stream st; ev (7 events) marks
+// the stage boundaries for the profiler.
+static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0, int nb, float* best,
+                       int32_t* trans, hipStream_t st, hipEvent_t* ev) {
     const int n = p->n;
     const size_t n3 = (size_t)n * n * n;
     const int nk = p->nkeys;
     const int nout = (p->mode == VX_MODE_PROB) ? (p->mix ? 3 : 2) : 1;
     const unsigned grp_mask = (p->mode == VX_MODE_PROB) ? 0x78u : 0u;
+#define VX_MARK(i) do { if (ev) VX_HIP(hipEventRecord(ev[i], st)); } while (0)
+    VX_MARK(0);
+    int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot + (size_t)r0 * 9, n, nb,
+                           (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, ws.q, ws.qd, ws.keys, nk, p->lo, p->w, st);
+    if (rc != VX_OK) return rc;
+    if (p->kind == KIND_DATA64) {
+        hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), nb), dim3(256), 0, st, ws.qd, ws.q, n);
+        VX_LAUNCH_CHECK();
+    }
+    VX_MARK(1);
+    rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st)
+                   : launch_fwd_zy(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt);
+    if (rc != VX_OK) return rc;
+    VX_MARK(2);
+    if (p->xvariant == 2) {
+        VX_REQUIRE(p->dft && p->dft_lo == p->lo && p->dft_w == p->w, VX_ERR_STATE,
+                   "MFMA x-pass matrices do not match the support box");
+        const unsigned short* mats[4] = {p->dft, p->dft + p->dft_elems[0], p->dft + 2 * p->dft_elems[0],
+                                         p->dft + 2 * p->dft_elems[0] + p->dft_elems[1]};
+        rc = launch_xcorr_mfma(ws.spec, p->tgt, ws.sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, p->xT, st);
+    } else {
+        rc = launch_xcorr(ws.spec, p->tgt, ws.sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
+                          p->xvariant, p->xT);
+    }
+    if (rc != VX_OK) return rc;
+    VX_MARK(3);
+    float* ro = (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
+    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st)
+                   : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt);
+    if (rc != VX_OK) return rc;
+    VX_MARK(4);
+    if (p->mode == VX_MODE_PROB && p->mix) {
+        const float wa = (float)(1.0 - p->alpha), wb = (float)p->alpha;
+        unsigned gx = (unsigned)std::min<size_t>((n3 + 255) / 256, 512);
+        hipLaunchKernelGGL(k_mixed, dim3(gx, nb), dim3(256), 0, st, ws.real_out, n3,
+                           (float)p->vave, (float)p->vstd, (float)p->pave, (float)p->pstd, wa, wb,
+                           ws.keys, nk);
+        VX_LAUNCH_CHECK();
+    }
+    VX_MARK(5);
+    hipLaunchKernelGGL(k_finalize, dim3((nb * nout + 255) / 256), dim3(256), 0, st, ws.keys, nb, nk, nout,
+                       n, r0, best, trans);
+    VX_LAUNCH_CHECK();
+    VX_MARK(6);
+#undef VX_MARK
+    return VX_OK;
+}
+
+static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
+                           hipStream_t st, hipEvent_t* ev, double* stage_ms) {
+    VX_REQUIRE(p && rot && best && trans, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
+    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+    const size_t n3 = (size_t)p->n * p->n * p->n;
     if (p->q_dirty) {
         // the gather writes only the support box: clear the planes once
         VX_HIP(hipMemsetAsync(p->q, 0, (size_t)p->batch * p->C * n3 * sizeof(float), st));
         if (p->qd) VX_HIP(hipMemsetAsync(p->qd, 0, (size_t)p->batch * n3 * sizeof(double), st));
+        if (p->q2) VX_HIP(hipMemsetAsync(p->q2, 0, (size_t)p->batch * p->C * n3 * sizeof(float), st));
+        if (p->qd2) VX_HIP(hipMemsetAsync(p->qd2, 0, (size_t)p->batch * n3 * sizeof(double), st));
         p->q_dirty = false;
     }
-#define VX_MARK(i) do { if (ev) VX_HIP(hipEventRecord(ev[i], st)); } while (0)
-    for (int r0 = 0; r0 < nrows; r0 += p->batch) {
-        const int nb = std::min(p->batch, nrows - r0);
-        VX_MARK(0);
-        int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot + (size_t)r0 * 9, n, nb,
-                               (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, p->q, p->qd, p->keys, nk, p->lo, p->w, st);
-        if (rc != VX_OK) return rc;
-        if (p->kind == KIND_DATA64) {
-            hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), nb), dim3(256), 0, st, p->qd, p->q, n);
-            VX_LAUNCH_CHECK();
-        }
-        VX_MARK(1);
-        rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st)
-                       : launch_fwd_zy(p->q, p->spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt);
-        if (rc != VX_OK) return rc;
-        VX_MARK(2);
-        if (p->xvariant == 2) {
-            VX_REQUIRE(p->dft && p->dft_lo == p->lo && p->dft_w == p->w, VX_ERR_STATE,
-                       "MFMA x-pass matrices do not match the support box");
-            const unsigned short* mats[4] = {p->dft, p->dft + p->dft_elems[0], p->dft + 2 * p->dft_elems[0],
-                                             p->dft + 2 * p->dft_elems[0] + p->dft_elems[1]};
-            rc = launch_xcorr_mfma(p->spec, p->tgt, p->sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, p->xT, st);
-        } else {
-            rc = launch_xcorr(p->spec, p->tgt, p->sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
-                              p->xvariant, p->xT);
-        }
-        if (rc != VX_OK) return rc;
-        VX_MARK(3);
-        float* ro = (p->mode == VX_MODE_PROB && p->mix) ? p->real_out : nullptr;
-        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st)
-                       : launch_inv_yz(p->sum, p->fd, n, nb * p->G, p->G, p->keys, nk, ro, st, p->slab_nt);
-        if (rc != VX_OK) return rc;
-        VX_MARK(4);
-        if (p->mode == VX_MODE_PROB && p->mix) {
-            const float wa = (float)(1.0 - p->alpha), wb = (float)p->alpha;
-            unsigned gx = (unsigned)std::min<size_t>((n3 + 255) / 256, 512);
-            hipLaunchKernelGGL(k_mixed, dim3(gx, nb), dim3(256), 0, st, p->real_out, n3,
-                               (float)p->vave, (float)p->vstd, (float)p->pave, (float)p->pstd, wa, wb,
-                               p->keys, nk);
-            VX_LAUNCH_CHECK();
-        }
-        VX_MARK(5);
-        hipLaunchKernelGGL(k_finalize, dim3((nb * nout + 255) / 256), dim3(256), 0, st, p->keys, nb, nk, nout,
-                           n, r0, best, trans);
-        VX_LAUNCH_CHECK();
-        VX_MARK(6);
-        if (ev) {
-            VX_HIP(hipEventSynchronize(ev[kStages]));
-            for (int i = 0; i < kStages; ++i) {
-                float ms = 0.f;
-                VX_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
-                stage_ms[i] += ms;
+    const Workspace ws0 = {p->q, p->qd, p->spec, p->sum, p->keys, p->real_out};
+    const int nbatch = (nrows + p->batch - 1) / p->batch;
+    if (ev || p->nstreams < 2 || nbatch < 2) {
+        for (int r0 = 0; r0 < nrows; r0 += p->batch) {
+            const int nb = std::min(p->batch, nrows - r0);
+            int rc = score_batch(p, ws0, rot, r0, nb, best, trans, st, ev);
+            if (rc != VX_OK) return rc;
+            if (ev) {
+                VX_HIP(hipEventSynchronize(ev[kStages]));
+                for (int i = 0; i < kStages; ++i) {
+                    float ms = 0.f;
+                    VX_HIP(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+                    stage_ms[i] += ms;
+                }
             }
         }
+        return VX_OK;
     }
-#undef VX_MARK
+    // Two batches in flight: even batches on the caller's stream with the
+    // first workspace, odd ones on the plan's second stream with the second
+    // (fork/join through events, so the caller's stream orders everything
+    // and graph capture records the fork).
+    const Workspace ws1 = {p->q2, p->qd2, p->spec2, p->sum2, p->keys2, p->real_out2};
+    VX_HIP(hipEventRecord(p->ev_fork, st));
+    VX_HIP(hipStreamWaitEvent(p->st2, p->ev_fork, 0));
+    for (int i = 0; i < nbatch; ++i) {
+        const int r0 = i * p->batch;
+        const int nb = std::min(p->batch, nrows - r0);
+        int rc = (i & 1) ? score_batch(p, ws1, rot, r0, nb, best, trans, p->st2, nullptr)
+                         : score_batch(p, ws0, rot, r0, nb, best, trans, st, nullptr);
+        if (rc != VX_OK) return rc;
+    }
+    VX_HIP(hipEventRecord(p->ev_join, p->st2));
+    VX_HIP(hipStreamWaitEvent(st, p->ev_join, 0));
     return VX_OK;
 }
 

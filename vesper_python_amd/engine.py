@@ -178,7 +178,7 @@ class SearchPlan:
                     box_width=int(out[7]))
 
     def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None,
-             pencil: bool | None = None, slab_threads: int | None = None):
+             pencil: bool | None = None, slab_threads: int | None = None, streams: int | None = None):
         """Select kernel schedules.  x-pass variants 0, 1 and 3 (FFT), the slab
         shape and the pencil switch give identical results; variant 2
         (split-bf16 MFMA DFT) agrees with them within 1e-4 relative."""
@@ -190,6 +190,8 @@ class SearchPlan:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_XPASS_TILE, int(xpass_tile))
         if slab_threads is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_SLAB_THREADS, int(slab_threads))
+        if streams is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_STREAMS, int(streams))
 
     def pin_box(self, lo: int, width: int):
         """Fix the support box to [lo, lo + width) on every axis (width 0 =
