@@ -60,9 +60,13 @@ __device__ __forceinline__ FftDesc stage_twiddles(const FftDesc& d, float2* lds_
 // Quotient of small non-negative integers (a < 2^22, d < 2^16) through a
 // float reciprocal, corrected to exact: avoids the ~40-instruction integer
 // division in butterfly index math.
+// Index products below fit 24 bits (offsets < 2^22): the full-rate 24-bit
+// multiply replaces the quarter-rate 32-bit one.
+__device__ __forceinline__ int imul(int a, int b) { return __mul24(a, b); }
+
 __device__ __forceinline__ int fdiv(int a, int d, float dinv) {
     int q = (int)((float)a * dinv);
-    const int r = a - q * d;
+    const int r = a - imul(q, d);
     q += (r >= d) ? 1 : ((r < 0) ? -1 : 0);
     return q;
 }
@@ -202,23 +206,24 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
         obase[m] = -1;
         if (b < nb) {
             int line, i;
-            if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
-            else           { line = fdiv(b, T, tinv);   i = b - line * T; }
-            float2* L = buf + line * ls;
+            if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
+            else           { line = fdiv(b, T, tinv);   i = b - imul(line, T); }
+            float2* L = buf + imul(line, ls) + imul(i, es);
+            const int tes = imul(T, es);
 #pragma unroll
-            for (int j = 0; j < R; ++j) v[m][j] = L[(i + j * T) * es];
-            const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
+            for (int j = 0; j < R; ++j) v[m][j] = L[j * tes];
+            const int k = (p == 1) ? 0 : (i - imul(fdiv(i, p, pinv), p));
             if (k != 0) {
                 const float2* w = tw + k;
 #pragma unroll
                 for (int j = 1; j < R; ++j) v[m][j] = cmul(v[m][j], w[(j - 1) * p]);
             }
             dft<R>(v[m]);
-            obase[m] = line * ls + ((i - k) * R + k) * es;
+            obase[m] = imul(line, ls) + imul(imul(i - k, R) + k, es);
         }
     }
     __syncthreads();
-    const int pes = p * es;
+    const int pes = imul(p, es);
 #pragma unroll
     for (int m = 0; m < MAXB; ++m) {
         if (obase[m] >= 0) {
@@ -255,21 +260,22 @@ __device__ __forceinline__ void butterfly_pp(const float2* __restrict__ in, floa
                                              int nlines, float linv, float tinv, int es, int ls, bool line_fast,
                                              int p, float pinv, const float2* __restrict__ tw) {
     int line, i;
-    if (line_fast) { i = fdiv(b, nlines, linv); line = b - i * nlines; }
-    else           { line = fdiv(b, T, tinv);   i = b - line * T; }
-    const float2* L = in + line * ls;
+    if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
+    else           { line = fdiv(b, T, tinv);   i = b - imul(line, T); }
+    const float2* L = in + imul(line, ls) + imul(i, es);
+    const int tes = imul(T, es);
     float2 v[R];
 #pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = L[(i + j * T) * es];
-    const int k = (p == 1) ? 0 : (i - fdiv(i, p, pinv) * p);
+    for (int j = 0; j < R; ++j) v[j] = L[j * tes];
+    const int k = (p == 1) ? 0 : (i - imul(fdiv(i, p, pinv), p));
     if (k != 0) {
         const float2* w = tw + k;
 #pragma unroll
         for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[(j - 1) * p]);
     }
     dft<R>(v);
-    float2* O = out + line * ls + ((i - k) * R + k) * es;
-    const int pes = p * es;
+    float2* O = out + imul(line, ls) + imul(imul(i - k, R) + k, es);
+    const int pes = imul(p, es);
 #pragma unroll
     for (int j = 0; j < R; ++j) O[j * pes] = v[j];
 }
