@@ -425,8 +425,9 @@ def test_fft_search_score_trans_1d_modes(gpu, mode):
 
 def test_slab_shapes_identical_at_n175(gpu):
     """N = 175 (radices 5 and 7): the 1024-thread slab kernels compiled for
-    those radices and the generic 512-thread ones give bit-identical scores,
-    and the rows match the FP64 oracle."""
+    those radices and the generic 512-thread ones, and the x pass in 256-,
+    128- and 64-thread workgroups (tiles of 8, 4 and 2 columns), give
+    bit-identical scores, and the rows match the FP64 oracle."""
     n = 175
     rng = np.random.default_rng(1750)
     c = n / 2
@@ -439,15 +440,16 @@ def test_slab_shapes_identical_at_n175(gpu):
     angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 170]], np.float64)
     rot = torch.from_numpy(rotation_rows(angles)).cuda()
     res = []
-    for nt in (0, 512):
-        _say("slab shape n=175", nt)
+    for nt, tile in ((0, 8), (512, 8), (0, 4), (0, 2)):
+        _say("slab shape n=175", nt, "x tile", tile)
         plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=2)
-        plan.tune(slab_threads=nt)
+        plan.tune(slab_threads=nt, xpass_variant=3, xpass_tile=tile)
         plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32))
         plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32))
         b, t = plan.score_rows(rot)
         res.append((b.cpu().numpy(), t.cpu().numpy()))
-    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    for b, t in res[1:]:
+        assert np.array_equal(res[0][0], b) and np.array_equal(res[0][1], t)
     o32, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref32")
     o64, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref64")
     _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=175")
@@ -472,3 +474,21 @@ def test_xpass_variants_identical(gpu, n, mode):
         res.append((b.cpu().numpy(), t.cpu().numpy()))
     for b, t in res[1:]:
         assert np.array_equal(b, res[0][0]) and np.array_equal(t, res[0][1])
+
+
+def test_correlate_through_pencils_n250(gpu):
+    """vx_correlate at N = 250, where the slab exceeds LDS and the pencil
+    passes run: one channel's correlation map within 1e-4 of the FP64
+    oracle's largest value."""
+    from vesper_python_amd import VESPER_1_prob_new as V
+    n = 250
+    rng = np.random.default_rng(2500)
+    t = np.zeros((n, n, n), np.float32)
+    q = np.zeros((n, n, n), np.float32)
+    t[90:150, 100:160, 80:140] = rng.standard_normal((60, 60, 60)) * (rng.uniform(size=(60, 60, 60)) < 0.2)
+    q[100:140, 95:135, 110:150] = rng.standard_normal((40, 40, 40)) * (rng.uniform(size=(40, 40, 40)) < 0.2)
+    spec = ref.target_spectra([t])
+    _say("correlate pencils", n)
+    got = V.fft_search_best_dot(spec, [q], None, None, None, None, None)[0]
+    exp = ref.fft_correlate(spec, [q], "ref64")[0]
+    assert np.max(np.abs(got - exp)) <= RTOL * np.max(np.abs(exp))

@@ -31,7 +31,7 @@ def candidates(n: int, groups: int = 1):
     out = [dict(xpass_variant=0, xpass_tile=0)]
     for t in (2, 4, 8, 12, 16):
         out.append(dict(xpass_variant=1, xpass_tile=t))
-    for t in (4, 8, 12):
+    for t in (1, 2, 4, 8, 12):
         if n * t <= 2048:
             out.append(dict(xpass_variant=3, xpass_tile=t))
     if groups == 1 and n <= 192:

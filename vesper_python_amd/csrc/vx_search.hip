@@ -737,8 +737,8 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 // tile elements in every phase, so the group sums live in registers and the
 // workgroup needs two tile buffers of LDS instead of 2 + G (more workgroups
 // per CU).  Same arithmetic, in the same order.
-template <int G, int PF, unsigned RM>
-__global__ void __launch_bounds__(256)
+template <int G, int PF, unsigned RM, int NT>
+__global__ void __launch_bounds__(NT)
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
     extern __shared__ float2 lds[];
@@ -755,7 +755,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     int off[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
-        const int e = threadIdx.x + u * 256;
+        const int e = threadIdx.x + u * NT;
         off[u] = -1;
         if (e < ne) {
             const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
@@ -778,7 +778,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     for (int c = 0; c < C; ++c) {
 #pragma unroll
         for (int u = 0; u < PF; ++u)
-            if (off[u] >= 0) A[threadIdx.x + u * 256] = sv[u];
+            if (off[u] >= 0) A[threadIdx.x + u * NT] = sv[u];
         const float2* tc = tgt + (size_t)c * n * plane + j0;
 #pragma unroll
         for (int u = 0; u < PF; ++u)
@@ -796,7 +796,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
         const int g = G == 1 ? 0 : (int)((grp_mask >> c) & 1);
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + u * 256;
+            const int e = threadIdx.x + u * NT;
             if (off[u] >= 0) {
                 const float2 pr = cmul(r[e], tv[u]);
                 if (G == 1 || g == 0) acc[0][u] = cadd(acc[0][u], pr);
@@ -809,7 +809,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     for (int g = 0; g < G; ++g) {
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + u * 256;
+            const int e = threadIdx.x + u * NT;
             if (off[u] >= 0) A[e] = cconj(acc[g][u]);
         }
         __syncthreads();
@@ -817,7 +817,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
         float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + u * 256;
+            const int e = threadIdx.x + u * NT;
             if (off[u] >= 0) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
         }
         __syncthreads();
@@ -1571,15 +1571,30 @@ static void launch_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec,
                        C, T, ntiles, grp_mask, lo, w);
 }
 
+template <int G, unsigned RM, int NT>
+static void launch_reg_pf_nt(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
+                             float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
+                             unsigned grp_mask, int lo, int w) {
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, kXcorrPF, RM, NT>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL((k_xcorr_reg_pf<G, kXcorrPF, RM, NT>), grid, dim3(NT), lds, st, spec, tgt, out, fd, n, nhz,
+                       nb, C, T, ntiles, grp_mask, lo, w);
+}
+
+// Smallest workgroup (64, 128 or 256 threads) holding a tile of n * T values
+// at kXcorrPF per thread: tiles of few columns run as one- or two-wave
+// workgroups whose barriers stall only their own waves.
 template <int G, unsigned RM>
 static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                           float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
                           unsigned grp_mask, int lo, int w) {
-    if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, kXcorrPF, RM>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_xcorr_reg_pf<G, kXcorrPF, RM>), grid, dim3(256), lds, st, spec, tgt, out, fd, n, nhz, nb,
-                       C, T, ntiles, grp_mask, lo, w);
+    if (n * T <= 64 * kXcorrPF)
+        launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    else if (n * T <= 128 * kXcorrPF)
+        launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    else
+        launch_reg_pf_nt<G, RM, 256>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
