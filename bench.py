@@ -55,6 +55,8 @@ def _stage_bytes(n: int, w: int, b: int) -> dict:
             "inv_yz": b * nh * 8}
 
 
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "xcorr_pair200_pmc.md")
 
 
@@ -225,6 +227,8 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a fill and a scale of this many MiB (counter calibration)")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
+    ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
+                    'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",
@@ -258,7 +262,11 @@ def main():
     plan.set_search(sdata, svec, None, n1.ave)
     rot = torch.from_numpy(rotation_rows(angles)).to(dev)
     tuned = None
-    if not args.no_tune:
+    if args.schedule:
+        sched = json.loads(args.schedule)
+        plan.tune(**sched)
+        tuned = {"schedule": sched}
+    elif not args.no_tune:
         from vesper_python_amd import autotune
         # strided sample of the real rows; the schedule choice is cached per (device, N, mode, batch)
         tuned = autotune.tune(plan, rot[::max(1, R // 256)].contiguous(), sample_rows=128, use_cache=False,
@@ -314,6 +322,17 @@ def main():
     roofline["path"] = {"bytes_per_row": row_bytes, "achieved": round(value * row_bytes / 1e9, 1),
                         "frac": round(value * row_bytes / 1e9 / PEAK_HBM_GBS, 4),
                         "model": "16 N^3 + 72 N^2 (N/2+1) bytes per row"}
+    sched = tuned["schedule"] if tuned else {}
+    if isinstance(sched, dict) and sched.get("xpass_variant") == 2:
+        # split-bf16 MFMA x pass: 3 products per real-form complex GEMM,
+        # forward [2NP x 2KP] x [2KP x plane] per channel, inverse [2NP x 2NP]
+        npad = 16 * ((n + 15) // 16)
+        kp = 16 * ((info["box_width"] + 15) // 16)
+        plane = n * (n // 2 + 1)
+        flops = 2 * 3 * args.batch * plane * (3 * (2 * npad) * (2 * kp) + (2 * npad) ** 2)
+        tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
+        roofline["mfma"] = {"kernel": "xcorr", "flops_per_launch": flops, "achieved": round(tflops, 1),
+                            "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
     if dom == "xcorr":
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
