@@ -1,4 +1,4 @@
-"""Captured search steps (HIP graphs) and the multi-pair driver: a replayed
+"""Captured search steps (HIP graphs) and the multi-pair search: a replayed
 step returns the eager step's results bit for bit, replay after replay with
 eager work in between, and pairs of uneven grid size searched through
 per-edge graphs match the per-pair eager search and the oracle."""

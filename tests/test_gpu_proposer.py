@@ -2,7 +2,8 @@
 transposes and epilogues), MLP gradients from the library's backward kernels
 against the FP64 oracle with the same bf16 rounding points, training, the
 moment descriptors, and the proposal's top-10 recall against the exhaustive
-search (its judge) compared with random rows of the same budget."""
+search (the reference for recall) compared with random rows of the same
+budget."""
 import contextlib
 import io
 
@@ -148,11 +149,11 @@ def test_proposal_recall_beats_random(gpu, tmp_path):
     assert np.mean(rec_mlp) >= 0.5
 
 
-def test_proposal_at_a5_judged_by_exhaustive_search(gpu, tmp_path):
+def test_proposal_at_a5_against_exhaustive_search(gpu, tmp_path):
     """The learned-proposal protocol at -A 5 (191,808 rows): the network is
     trained on the pair's own exhaustive -A 10 landscape, 5 % of the -A 5 rows
-    are FFT-scored in two rounds, and the exhaustive -A 5 search judges the
-    top-10 recall; random rows of the same budget are the baseline."""
+    are FFT-scored in two rounds, and recall is measured against the
+    exhaustive -A 5 top-10; random rows of the same budget are the baseline."""
     import time
     fine = calc_angle_comb(5.0)
     coarse = calc_angle_comb(10.0)

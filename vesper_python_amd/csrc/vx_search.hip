@@ -350,12 +350,13 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
         float2 A[MU], B[MU];
         int dst[MU];
         const int nu = npair * nhz;
+        const float nhzinv = 1.0f / (float)nhz;
 #pragma unroll
         for (int m = 0; m < MU; ++m) {
             const int u = threadIdx.x + m * nthr;
             dst[m] = -1;
             if (u < nu) {
-                const int l = u / nhz, k = u - l * nhz;
+                const int l = fdiv(u, nhz, nhzinv), k = u - l * nhz;
                 A[m] = make_float2(0.f, 0.f);
                 B[m] = make_float2(0.f, 0.f);
                 if (l >= l0 && l < l1) {
@@ -1154,6 +1155,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
     const int npair = (n + 1) >> 1;
     const int half = n >> 1;
     const bool even = (n & 1) == 0;
+    const float ninv_p = 1.0f / (float)n;
     // pack conj(Z), Z[k] = A[k] + i B[k] with A, B = rows 2l, 2l+1 after the
     // y-inverse (lds holds r = conj of them); Hermitian extension for k > n/2;
     // imaginary parts of the DC (and Nyquist) bins dropped as c2r does.
@@ -1167,7 +1169,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
             const int u = threadIdx.x + m * nthr;
             dst[m] = -1;
             if (u < nu) {
-                const int l = u / n, k = u - l * n;
+                const int l = fdiv(u, n, ninv_p), k = u - l * n;
                 const bool hasb = (2 * l + 1) < n;
                 if (k <= half) {
                     float2 ra = lds[(2 * l) * nhz + k];
@@ -1195,7 +1197,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
     uint32_t bi = 0xffffffffu;
     const size_t n3 = (size_t)n * n * n;
     for (int e = threadIdx.x; e < npair * n; e += nthr) {
-        const int l = e / n, z = e - l * n;
+        const int l = fdiv(e, n, ninv_p), z = e - l * n;
         const float2 f = lds[e];
         const uint32_t ia = ((uint32_t)x * n + 2 * l) * n + z;
         const float va = f.x;

@@ -8,8 +8,8 @@ library's own forward/backward kernels and Adam on score landscapes the
 exhaustive search produces: across training pairs, and per query pair on a
 coarse landscape (every row of a coarser angle step) before proposing rows of
 the fine step.  Only the predicted top fraction of fine rows is FFT-scored;
-the exhaustive search stays available as the fallback and as the judge
-(top-10 recall).  No reference counterpart: the reference scores every row
+the exhaustive search stays available as the fallback and as the
+reference its top-10 recall is measured against.  No reference counterpart: the reference scores every row
 (VESPER_1_prob_new.py:644-655).
 """
 from __future__ import annotations
