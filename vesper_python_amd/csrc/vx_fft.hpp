@@ -198,6 +198,7 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
     const int nb = T * nlines;
     const int nthr = blockDim.x;
     const float linv = 1.0f / (float)nlines, tinv = 1.0f / (float)T;
+    const int lg = (nlines & (nlines - 1)) == 0 ? __ffs(nlines) - 1 : -1;
     float2 v[MAXB][R];
     int obase[MAXB];
 #pragma unroll
@@ -206,7 +207,8 @@ __device__ __forceinline__ void fft_stage(float2* buf, int n, int nlines, int es
         obase[m] = -1;
         if (b < nb) {
             int line, i;
-            if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
+            if (line_fast && lg >= 0) { i = b >> lg; line = b & (nlines - 1); }
+            else if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
             else           { line = fdiv(b, T, tinv);   i = b - imul(line, T); }
             float2* L = buf + imul(line, ls) + imul(i, es);
             const int tes = imul(T, es);
@@ -258,9 +260,10 @@ __device__ void fft_lds(float2* buf, const FftDesc& d, int nlines, int es, int l
 template <int R>
 __device__ __forceinline__ void butterfly_pp(const float2* __restrict__ in, float2* __restrict__ out, int b, int T,
                                              int nlines, float linv, float tinv, int es, int ls, bool line_fast,
-                                             int p, float pinv, const float2* __restrict__ tw) {
+                                             int p, float pinv, const float2* __restrict__ tw, int lg) {
     int line, i;
-    if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
+    if (line_fast && lg >= 0) { i = b >> lg; line = b & (nlines - 1); }   // power-of-two line count
+    else if (line_fast) { i = fdiv(b, nlines, linv); line = b - imul(i, nlines); }
     else           { line = fdiv(b, T, tinv);   i = b - imul(line, T); }
     const float2* L = in + imul(line, ls) + imul(i, es);
     const int tes = imul(T, es);
@@ -288,8 +291,9 @@ __device__ __forceinline__ void fft_stage_pp(const float2* __restrict__ in, floa
     const int nb = T * nlines;
     const int nthr = blockDim.x;
     const float linv = 1.0f / (float)nlines, tinv = 1.0f / (float)T;
+    const int lg = (nlines & (nlines - 1)) == 0 ? __ffs(nlines) - 1 : -1;
     for (int b = threadIdx.x; b < nb; b += nthr)
-        butterfly_pp<R>(in, out, b, T, nlines, linv, tinv, es, ls, line_fast, p, pinv, tw);
+        butterfly_pp<R>(in, out, b, T, nlines, linv, tinv, es, ls, line_fast, p, pinv, tw, lg);
     __syncthreads();
 }
 
