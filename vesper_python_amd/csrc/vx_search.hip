@@ -204,12 +204,10 @@ k_gather(const float4* __restrict__ svec, const float2* __restrict__ sscal,
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < nkeys) keys[(size_t)b * nkeys + threadIdx.x] = 0ull;
     // destination box [lo, lo + w)^3 holds every voxel that can receive a
     // non-zero value; the planes are zero outside it
-    // one thread per (y, z) of the box's x-plane blockIdx.y, all lanes busy
     const int x = lo + blockIdx.y;
-    const int t = blockIdx.x * 256 + threadIdx.x;
-    if (t >= w * w) return;
-    const int yy = fdiv(t, w, 1.0f / (float)w);
-    const int y = lo + yy, z = lo + (t - yy * w);
+    const int y = lo + (blockIdx.x / nzb) * 4 + (threadIdx.x >> 6);
+    const int z = lo + (blockIdx.x % nzb) * 64 + (threadIdx.x & 63);
+    if (y >= lo + w || z >= lo + w) return;
     float F[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) F[i] = rot[b * 9 + i];
@@ -1677,8 +1675,8 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
 int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
                   const double* sdat, const float* rot, int n, int nb, int fma, float* q,
                   double* qd, unsigned long long* keys, int nkeys, int lo, int w, hipStream_t st) {
-    const int nzb = 0;   // unused by the flattened (y, z) mapping
-    dim3 grid((unsigned)((w * w + 255) / 256), w, nb);
+    const int nzb = (w + 63) / 64;
+    dim3 grid(nzb * ((w + 3) / 4), w, nb);
     switch (kind) {
         case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
         case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
