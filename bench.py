@@ -186,6 +186,48 @@ def cpu_baseline_pool(n1, n2, angles, budget_s: float, workers: int):
                        f"processes x 1 thread, oracle ref32, {dt:.1f} s")
 
 
+def measure_prob(n1, n2, rot, batch, steps):
+    """Prob-mode search speed on the same pair (7 channels: 3 vector + 4
+    probability, two correlation groups, the alpha-mixed z-score fused into
+    the search): rows/s of the mixing pass (the heavier of the two passes).
+    The probability channels are synthetic: four random fractions that sum to
+    one on the voxels where each map's density is non-zero."""
+    import torch
+    from vesper_python_amd import _native as nat
+    from vesper_python_amd import engine
+
+    def probs(m, seed):
+        g = torch.Generator(device="cuda").manual_seed(seed)
+        d = m._dev[0]
+        u = torch.rand((4,) + tuple(d.shape), generator=g, device="cuda", dtype=torch.float32) + 1e-3
+        u = u / u.sum(dim=0, keepdim=True)
+        return (u * (d != 0).float()).contiguous()
+
+    n = int(n1.xdim)
+    plan = engine.SearchPlan(n, nat.VX_MODE_PROB, batch=batch)
+    plan.set_target(*n1._dev, probs(n1, 1), 0.0)
+    plan.set_search(*n2._dev, probs(n2, 2), 0.0)
+    plan.set_prob_stats(0.5, 1.0, 0.0, 1.0, 0.0, enable=True)
+    from vesper_python_amd import autotune
+    R = int(rot.shape[0])
+    tuned = autotune.tune(plan, rot[::max(1, R // 256)].contiguous(), sample_rows=128, use_cache=False)
+    plan.score_rows(rot[: 2 * batch])
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        plan.score_rows(rot)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nprof = 8 * batch
+    stage = plan.profile_rows(rot[:nprof])
+    return {"rows_per_s": round(R / (ms / 1e3), 1), "ms_per_pass": round(ms, 3), "rows": R, "batch": batch,
+            "channels": 7, "pass": "mixing pass (vector, probability and mixed scores per row)",
+            "schedule": tuned["schedule"], "stage_ms_per_launch": {k: round(v / 8, 4) for k, v in stage.items()}}
+
+
 def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
     """Learned proposal on the same pair: train on the -A 30 landscape, then
     FFT-score 5 % of the -A `ang` rows in two rounds with retraining; recall
@@ -227,6 +269,7 @@ def main():
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="first run a fill and a scale of this many MiB (counter calibration)")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
+    ap.add_argument("--no-prob", action="store_true", help="skip the prob-mode measurement")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -337,6 +380,10 @@ def main():
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
 
+    prob = None
+    if not args.no_prob and not args.rows:
+        prob = measure_prob(n1, n2, rot, max(8, args.batch // 2), 1)
+
     proposal = None
     if not args.no_proposal and not args.rows:
         proposal = measure_proposal(plan, n1, n2, w["A"], best, R, value)
@@ -358,7 +405,8 @@ def main():
            "config": {"workload": f"{args.workload}: {w['text']}", "grid_edge": n, "rows": R,
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
-           "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal}
+           "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
+           "prob_mode": prob}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     if cpu_all:
