@@ -82,3 +82,28 @@ def test_cli_orig_runs(gpu, pair_files):
     out = buf.getvalue()
     assert len(res) == 5
     assert "###Start Searching###" in out and "#MAP SUM=" in out and "Nvox= 24, 24, 24" in out
+
+
+def test_cli_eval_mode_and_pdb_output(gpu, pair_files, tmp_path, monkeypatch):
+    """-E scores the maps as placed (get_score at translation 0, no search);
+    -S writes one CA/CB pseudo-atom PDB per refined model."""
+    paths, p = pair_files
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        res = cli.main(["orig", "-a", paths["A"], "-b", paths["A"], "-t", str(p.a.threshold),
+                        "-T", str(p.a.threshold), "-s", "7", "-E", "True"])
+    out = buf.getvalue()
+    assert res is None and "#For Evaluation Mode" in out and "Overlap=" in out and "Score=" in out
+    # the same map at translation 0: every voxel overlaps
+    ov = [l for l in out.splitlines() if l.startswith("Overlap=")][0]
+    nm, total = ov.split()[1].split("/")
+    assert nm == total
+    monkeypatch.chdir(tmp_path)
+    with contextlib.redirect_stdout(io.StringIO()):
+        res = cli.main(["orig", "-a", paths["A"], "-b", paths["B"], "-t", str(p.a.threshold),
+                        "-T", str(p.b.threshold), "-s", "7", "-A", "30", "-N", "3", "-S", "True",
+                        "-o", "pdb_out"])
+    pdbs = sorted((tmp_path / "pdb_out").glob("*.pdb"))
+    assert len(res) == 3 and len(pdbs) == 3
+    lines = pdbs[0].read_text().splitlines()
+    assert lines and all(l.startswith("ATOM") for l in lines) and " CA  ALA" in lines[0] and " CB  ALA" in lines[1]
