@@ -188,3 +188,50 @@ def test_proposal_at_a5_against_exhaustive_search(gpu, tmp_path):
               "exhaustive s", round(t_exh, 3), "proposal s", round(t_prop, 3), flush=True)
     assert np.mean(rec) > np.mean(rec_rand)
     assert np.mean(rec) >= 0.5
+
+
+def test_pretraining_across_pairs_then_held_out_pairs(gpu, tmp_path):
+    """The network trained across pairs on their exhaustive -A 10 landscapes
+    (the protocol's training set, scaled down to 6 pairs) proposes rows of
+    held-out pairs: pretrained alone, and pretrained then refined on the
+    held-out pair's own -A 30 landscape in the proposal rounds; top-10 recall
+    against each held-out pair's exhaustive search, random rows as baseline."""
+    fine = calc_angle_comb(10.0)
+    coarse = calc_angle_comb(30.0)
+    train = []
+    for seed in (5101, 5102, 5103, 5104, 5105, 5106):
+        n1, n2 = _pair(tmp_path, seed)
+        plan = engine.SearchPlan(n1.xdim, nat.VX_MODE_VEC, batch=16)
+        plan.set_target(*n1._dev, None, n1.ave)
+        plan.set_search(*n2._dev, None, n1.ave)
+        train.append(proposer.Proposer(plan, *n1._dev, *n2._dev, None))
+    mlp = proposer.RotationMLP(seed=11)
+    losses = proposer.pretrain(mlp, train, fine, epochs=8, lr=2e-3)
+    print("[pretrain] loss", losses[0], "->", losses[-1], flush=True)
+    assert losses[-1] < losses[0]
+    st = mlp.state()
+    rng = np.random.default_rng(2)
+    r_pre, r_ft, r_rand = [], [], []
+    for seed in (5001, 5003):
+        n1, n2 = _pair(tmp_path, seed)
+        plan = engine.SearchPlan(n1.xdim, nat.VX_MODE_VEC, batch=16)
+        plan.set_target(*n1._dev, None, n1.ave)
+        plan.set_search(*n2._dev, None, n1.ave)
+        best, _ = plan.score_rows(torch.from_numpy(rotation_rows(fine)).cuda())
+        truth = engine.topn(best[:, 0].contiguous(), 10)
+        m = proposer.RotationMLP(seed=12)
+        m.load(st)
+        top, _, scored = proposer.Proposer(plan, *n1._dev, *n2._dev, m).propose(fine, frac=0.05)
+        r_pre.append(proposer.recall(top, truth))
+        m2 = proposer.RotationMLP(seed=13)
+        m2.load(st)
+        top2, scored2, _ = proposer.propose_rounds(proposer.Proposer(plan, *n1._dev, *n2._dev, m2), coarse, fine,
+                                                   frac=0.05, rounds=2, epochs=10)
+        r_ft.append(proposer.recall(top2, truth))
+        rand = rng.choice(len(fine), scored, replace=False)
+        b = best[:, 0].cpu().numpy()[rand]
+        r_rand.append(proposer.recall(rand[np.argsort(-b, kind="stable")[:10]], truth))
+        print("[held-out]", seed, "pretrained", r_pre[-1], "pretrained+rounds", r_ft[-1], "random", r_rand[-1],
+              flush=True)
+    assert np.mean(r_ft) > np.mean(r_rand)
+    assert np.mean(r_ft) >= 0.5

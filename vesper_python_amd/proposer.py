@@ -222,6 +222,25 @@ class Proposer:
         return sel[torch.from_numpy(t.astype(np.int64)).to(sel.device)].cpu().numpy(), b, int(sel.numel())
 
 
+def pretrain(mlp: RotationMLP, pairs, angles, epochs=20, lr=2e-3, batch=2048, seed=0):
+    """Train `mlp` across pairs: every row of `angles` of each pair (a
+    Proposer holding the pair's plan and maps) is FFT-scored exhaustively and
+    z-scored within its pair; the rows of all pairs form one training set.
+    Returns the per-epoch losses."""
+    Xs, ys = [], []
+    for pr in pairs:
+        rot, X = pr.rows_features(angles)
+        R = int(rot.shape[0])
+        Xs.append(X[:R])
+        ys.append(zscore(landscape(pr.plan, rot)))
+    X = torch.cat(Xs).contiguous()
+    y = torch.cat(ys).contiguous()
+    M = int(y.numel())
+    Xp = torch.zeros((pad64(M), IN), dtype=X.dtype, device=X.device)
+    Xp[:M] = X
+    return mlp.fit(Xp, y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
+
+
 def grid_shape(angles):
     """(nx, nz) when `angles` is a calc_angle_comb grid (row = iz nx^2 + ix nx + iy
     for angle (x[ix], x[iy], z[iz])), else None."""
