@@ -228,6 +228,50 @@ def measure_prob(n1, n2, rot, batch, steps):
             "schedule": tuned["schedule"], "stage_ms_per_launch": {k: round(v / 8, 4) for k, v in stage.items()}}
 
 
+def measure_interp(edge: int = 337, cpu: bool = True):
+    """Prediction-grid interpolation (interpolation_functions.py:6-135) on a
+    synthetic [edge]^3 x 4 float32 grid with values on one parity lattice,
+    the size of the reference notebook's timing: GPU time of vx_interpolate
+    (HIP events, grid resident in HBM), and the oracle's vectorised NumPy
+    form on the host for comparison."""
+    import torch
+    from vesper_python_amd import interpolation as ip
+    g = torch.Generator(device="cuda").manual_seed(5)
+    d = torch.zeros((edge, edge, edge, 4), dtype=torch.float32, device="cuda")
+    d[::2, ::2, ::2] = torch.rand(d[::2, ::2, ::2].shape, generator=g, device="cuda") + 0.01
+    work = d.clone()
+    ip.interpolate_device(work)                     # warm-up
+    torch.cuda.synchronize()
+    reps = 5
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    ms = 0.0
+    for _ in range(reps):
+        work.copy_(d)
+        e0.record()
+        ip.interpolate_device(work)
+        e1.record()
+        torch.cuda.synchronize()
+        ms += e0.elapsed_time(e1)
+    ms /= reps
+    grid_bytes = d.numel() * 4
+    model = grid_bytes + grid_bytes * 7 // 8        # read the grid, write the seven filled lattices
+    rec = {"grid": [edge, edge, edge, 4], "dtype": "f32", "gpu_ms": round(ms, 3),
+           "achieved_GBps": round(model / (ms / 1e3) / 1e9, 1), "bytes_model": model,
+           "peak_GBps": PEAK_HBM_GBS, "frac": round(model / (ms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)}
+    if cpu:
+        from threadpoolctl import threadpool_limits
+        from oracle import interp_ref as ir
+        host = d.cpu().numpy()
+        with threadpool_limits(1):
+            t0 = time.perf_counter()
+            ir.interpolate(host)
+            rec["cpu_port_s"] = round(time.perf_counter() - t0, 3)
+        rec["cpu_port"] = "oracle vectorised NumPy, 1 thread, same grid"
+        rec["gpu_matches_port"] = bool(np.array_equal(host, work.cpu().numpy()))
+    return rec
+
+
 def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
     """Learned proposal on the same pair: train on the -A 30 landscape, then
     FFT-score 5 % of the -A `ang` rows in two rounds with retraining; recall
@@ -270,6 +314,7 @@ def main():
                     help="first run a fill and a scale of this many MiB (counter calibration)")
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
     ap.add_argument("--no-prob", action="store_true", help="skip the prob-mode measurement")
+    ap.add_argument("--no-interp", action="store_true", help="skip the prediction-grid interpolation measurement")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -380,6 +425,10 @@ def main():
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
 
+    interp = None
+    if not args.no_interp and not args.rows:
+        interp = measure_interp(cpu=not args.no_cpu)
+
     prob = None
     if not args.no_prob and not args.rows:
         prob = measure_prob(n1, n2, rot, max(8, args.batch // 2), 1)
@@ -406,7 +455,7 @@ def main():
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
-           "prob_mode": prob}
+           "prob_mode": prob, "interpolation": interp}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     if cpu_all:

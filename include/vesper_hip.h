@@ -205,6 +205,18 @@ int vx_threshold_dmax(float* data_dev, int sx, int sy, int sz, float th, double*
 int vx_laplacian(int n, const double* in_dev, double* out_dev, vx_stream stream);
 
 /*
+ * interpolate (interpolation_functions.py:6-135): fill the zero entries off
+ * the reference parity lattice of a class-prediction grid [sx][sy][sz][nl]
+ * (float32, or float64 with is_f64 = 1), in place on the device.  The
+ * reference parity is that of the first voxel in C order whose channel 0 is
+ * positive (returned in ref_host3 when non-NULL); a grid without one is an
+ * error.  Seven passes as the reference's loops, with its index -1
+ * wrap-around and float64 averages.  Synchronises the stream once.
+ */
+int vx_interpolate(void* data_dev, int is_f64, int sx, int sy, int sz, int nl, int32_t* ref_host3,
+                   vx_stream stream);
+
+/*
  * fft_search_best_dot (VESPER_1_prob_new.py:464-491) and the transform of
  * fft_search_score_trans_1d (:529-565): for each of nch channels,
  * out_dev[c] = irfftn(tspec_dev[c] * rfftn(q_dev[c])), unnormalised, with the

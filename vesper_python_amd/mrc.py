@@ -51,16 +51,21 @@ def read_mrc(path: str) -> MrcMap:
     return MrcMap(data=data, voxel_size=vs, origin=origin, nx=nx, ny=ny, nz=nz)
 
 
-def write_mrc(path: str, data_zyx: np.ndarray, voxel_size=1.0, origin=(0.0, 0.0, 0.0)) -> None:
-    """Write a float32 MRC2014 file; data given in file order (nz, ny, nx)."""
+def write_mrc(path: str, data_zyx: np.ndarray, voxel_size=1.0, origin=(0.0, 0.0, 0.0), start=(0, 0, 0),
+              cella=None) -> None:
+    """Write a float32 MRC2014 file; data given in file order (nz, ny, nx).
+    `start` fills NXSTART/NYSTART/NZSTART; `cella` (x, y, z), when given,
+    replaces voxel_size x (nx, ny, nz) as the cell lengths."""
     arr = np.ascontiguousarray(data_zyx, dtype="<f4")
     nz, ny, nx = arr.shape
     vs = np.broadcast_to(np.asarray(voxel_size, dtype=np.float32), (3,))
+    cell = ([float(vs[0]) * nx, float(vs[1]) * ny, float(vs[2]) * nz] if cella is None
+            else [float(c) for c in np.asarray(cella, dtype=np.float32)])
     h = bytearray(1024)
     struct.pack_into("<4i", h, 0, nx, ny, nz, 2)
-    struct.pack_into("<3i", h, 16, 0, 0, 0)
+    struct.pack_into("<3i", h, 16, int(start[0]), int(start[1]), int(start[2]))
     struct.pack_into("<3i", h, 28, nx, ny, nz)
-    struct.pack_into("<3f", h, 40, float(vs[0]) * nx, float(vs[1]) * ny, float(vs[2]) * nz)
+    struct.pack_into("<3f", h, 40, *cell)
     struct.pack_into("<3f", h, 52, 90.0, 90.0, 90.0)
     struct.pack_into("<3i", h, 64, 1, 2, 3)
     struct.pack_into("<3f", h, 76, float(arr.min()), float(arr.max()), float(arr.mean()))
