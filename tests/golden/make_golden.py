@@ -109,6 +109,18 @@ def fastvec_case():
                         n=n, dest_width=w, dest_orig=do, dreso=dreso, data=d, vec=v)
 
 
+def interp_case():
+    from oracle import interp_ref as ir
+    rng = np.random.default_rng(51)
+    d = np.zeros((9, 8, 7, 4), np.float32)
+    sl = (slice(1, 9, 2), slice(0, 8, 2), slice(1, 7, 2))
+    d[sl] = (rng.uniform(0.01, 1.0, d[sl].shape) * (rng.uniform(size=d[sl].shape) < 0.8)).astype(np.float32)
+    d[1, 0, 1, 0] = 0.5
+    out = d.copy()
+    parity = ir.interpolate_loops(out)
+    np.savez_compressed(os.path.join(HERE, "interp_9x8x7.npz"), grid=d, out=out, parity=np.array(parity))
+
+
 def tables():
     fast = [ref.next_fast_len(k) for k in range(1, 401)]
     rows = {str(a): int(len(ref.calc_angle_comb(a))) for a in (30.0, 10.0, 5.0)}
@@ -125,5 +137,6 @@ if __name__ == "__main__":
     search_cases()
     get_score_case()
     fastvec_case()
+    interp_case()
     tables()
     print("fixtures written to", HERE)

@@ -67,3 +67,11 @@ def test_c_oracle_reproduces_fastvec_fixture(fastvec_ref):
                      float(g["dest_width"]), g["dest_orig"], float(g["dreso"]))
     assert np.array_equal(d, g["data"]) and np.array_equal(v, g["vec"])
     assert (d != 0).sum() > 10
+
+
+def test_oracle_reproduces_interp_fixture():
+    from oracle import interp_ref as ir
+    g = _load("interp_9x8x7.npz")
+    d = g["grid"].copy()
+    assert ir.interpolate(d) == tuple(g["parity"])
+    assert np.array_equal(d, g["out"])

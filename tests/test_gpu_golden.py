@@ -71,3 +71,11 @@ def test_fastvec_matches_fixture(gpu):
     assert np.array_equal(d != 0, g["data"] != 0)
     assert np.allclose(d, g["data"], rtol=1e-12, atol=1e-300)
     assert np.max(np.abs(v.cpu().numpy() - g["vec"])) <= 4e-7
+
+
+def test_interpolate_matches_fixture(gpu):
+    from vesper_python_amd import interpolation as ip
+    g = _load("interp_9x8x7.npz")
+    d = torch.from_numpy(g["grid"].copy()).cuda()
+    assert ip.interpolate_device(d) == tuple(int(v) for v in g["parity"])
+    assert np.array_equal(d.cpu().numpy(), g["out"])
