@@ -228,6 +228,47 @@ def measure_prob(n1, n2, rot, batch, steps):
             "schedule": tuned["schedule"], "stage_ms_per_launch": {k: round(v / 8, 4) for k, v in stage.items()}}
 
 
+def measure_library_fft(n1, n2, rot, batch, plan):
+    """The same correlation done with the vendor FFT library (rocFFT through
+    torch.fft) on already-rotated query channels: forward r2c of the three
+    channels of `batch` rows, multiply by the conjugated target spectra,
+    channel sum, c2r, per-row maximum.  A reference point for the fused
+    kernels; it leaves out the rotation and the support-box pruning, so it
+    is a lower bound on a library-FFT path's time."""
+    import torch
+    from vesper_python_amd import engine
+    n = int(n1.xdim)
+    dims = (-3, -2, -1)
+    q = torch.empty((batch, 3, n, n, n), device="cuda", dtype=torch.float32)
+    for b in range(batch):
+        _, ov, _ = engine.rotate_map(rot[b].cpu().numpy(), n2._dev[0], n2._dev[1])
+        q[b] = ov.permute(3, 0, 1, 2)
+    tv = n1._dev[1].permute(3, 0, 1, 2).contiguous()
+    tspec = torch.fft.rfftn(tv, dim=dims).conj()
+
+    def run():
+        s = (torch.fft.rfftn(q, dim=dims) * tspec).sum(dim=1)
+        # unnormalised inverse, as the reference's c2r
+        return torch.fft.irfftn(s, s=(n, n, n), dim=dims, norm="forward").reshape(batch, -1).amax(dim=1)
+
+    run()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(3):
+        best = run()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 3
+    # the fused path's scores for the same rows
+    fb, _ = plan.score_rows(rot[:batch])
+    rel = float(((best - fb[:, 0]).abs() / fb[:, 0].abs()).max())
+    return {"rows_per_s": round(batch / (ms / 1e3), 1), "ms_per_batch": round(ms, 3), "batch": batch,
+            "max_rel_diff_vs_fused": rel,
+            "what": "torch.fft (rocFFT) r2c x 3 channels, multiply, sum, c2r, max; rotation excluded"}
+
+
 def measure_interp(edge: int = 337, cpu: bool = True):
     """Prediction-grid interpolation (interpolation_functions.py:6-135) on a
     synthetic [edge]^3 x 4 float32 grid with values on one parity lattice,
@@ -315,6 +356,7 @@ def main():
     ap.add_argument("--no-proposal", action="store_true", help="skip the learned-proposal measurement")
     ap.add_argument("--no-prob", action="store_true", help="skip the prob-mode measurement")
     ap.add_argument("--no-interp", action="store_true", help="skip the prediction-grid interpolation measurement")
+    ap.add_argument("--no-libfft", action="store_true", help="skip the vendor-FFT pipeline reference point")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -425,6 +467,10 @@ def main():
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
 
+    libfft = None
+    if not args.no_libfft and not args.rows:
+        libfft = measure_library_fft(n1, n2, rot, args.batch, plan)
+
     interp = None
     if not args.no_interp and not args.rows:
         interp = measure_interp(cpu=not args.no_cpu)
@@ -455,7 +501,7 @@ def main():
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
-           "prob_mode": prob, "interpolation": interp}
+           "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     if cpu_all:
