@@ -324,6 +324,85 @@ __device__ __forceinline__ float2* fft_lds_pp(float2* a, float2* b, const FftDes
     return a;
 }
 
+// ---------------------------------------------------------------------------
+// Compile-time column-tile FFTs.
+//
+// A tile holds NL lines of length n as [element][line] (element stride NL,
+// line stride 1) and consecutive threads take consecutive lines, the layout
+// of the x pass.  Butterfly b = line + i NL then reads in[b + j T NL] and
+// writes out[b + (R - 1) NL (i - k) + j P NL].  When the tile width and the
+// plan's radix order are known at compile time (StaticTile<NL, R0, R1, ...>,
+// radices in the order fft_desc_create chooses), every stage's R, P, span T,
+// twiddle offset and butterfly count are constants: the per-thread butterfly
+// loop unrolls, i = b / NL and k = i mod P reduce to shifts and
+// multiply-shifts, the LDS offsets become instruction immediates, and the
+// twiddle multiply runs unconditionally for P > 1 (the k = 0 twiddles are
+// exactly 1, so the values match the runtime form).
+// ---------------------------------------------------------------------------
+
+template <int N, int NL, int NT, int R, int P, int TW>
+__device__ __forceinline__ void tile_stage_static(const float2* __restrict__ in, float2* __restrict__ out,
+                                                  const float2* __restrict__ tw) {
+    constexpr int T = N / R;
+    constexpr int NB = T * NL;
+#pragma unroll 1
+    for (int b0 = 0; b0 < NB; b0 += NT) {
+        const int b = b0 + (int)threadIdx.x;
+        if (NB % NT == 0 || b < NB) {
+            const int i = b / NL;
+            float2 v[R];
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = in[b + j * T * NL];
+            int k = 0;
+            if constexpr (P > 1) {
+                k = (int)((unsigned)i % (unsigned)P);
+                const float2* w = tw + TW + k;
+#pragma unroll
+                for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[(j - 1) * P]);
+            }
+            dft<R>(v);
+            float2* O = out + b + (R - 1) * NL * (i - k);
+#pragma unroll
+            for (int j = 0; j < R; ++j) O[j * P * NL] = v[j];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NL, int NT, int P, int TW, int R, int... Rest>
+__device__ __forceinline__ float2* tile_fft_static_rec(float2* a, float2* b, const float2* tw) {
+    tile_stage_static<N, NL, NT, R, P, TW>(a, b, tw);
+    if constexpr (sizeof...(Rest) == 0) return b;
+    else return tile_fft_static_rec<N, NL, NT, P * R, TW + (R - 1) * P, Rest...>(b, a, tw);
+}
+
+template <int NL, int... Rs>
+struct StaticTile {
+    static constexpr bool kStatic = true;
+    static constexpr int kN = (Rs * ...);
+    static constexpr int kNL = NL;
+    // Forward FFT of the NL lines of a column tile held in a (ping-pong with
+    // b) by an NT-thread block; ends synchronised and returns the buffer
+    // holding the result.
+    template <int NT>
+    __device__ __forceinline__ static float2* tile_fft(float2* a, float2* b, const float2* tw) {
+        return tile_fft_static_rec<kN, NL, NT, 1, 0, Rs...>(a, b, tw);
+    }
+};
+
+// The runtime-descriptor form (any length and tile width).
+struct DynamicTile {
+    static constexpr bool kStatic = false;
+    static constexpr int kN = 0;
+    static constexpr int kNL = 0;
+};
+
+template <class SP, unsigned RM, int NT>
+__device__ __forceinline__ float2* tile_fft(float2* a, float2* b, const FftDesc& fd, int nl) {
+    if constexpr (SP::kStatic) return SP::template tile_fft<NT>(a, b, fd.twid);
+    else return fft_lds_pp<RM>(a, b, fd, nl, nl, 1, true);
+}
+
 }  // namespace vx
 
 #pragma clang fp contract(off)

@@ -47,6 +47,16 @@ static bool factorize(int n, std::vector<int>& r) {
     return m == 1 && (int)r.size() <= kMaxStages;
 }
 
+// True when fft_desc_create's stage order for n is exactly Rs... (the
+// compile-time plan then reads the descriptor's twiddle table unchanged).
+template <int... Rs>
+static bool static_plan_matches(int n) {
+    std::vector<int> r;
+    if (!factorize(n, r)) return false;
+    const std::vector<int> want{Rs...};
+    return r == want;
+}
+
 int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
     std::vector<int> r;
     VX_REQUIRE(n >= 2 && factorize(n, r), VX_ERR_UNSUPPORTED,
@@ -738,8 +748,8 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 // tile elements in every phase, so the group sums live in registers and the
 // workgroup needs two tile buffers of LDS instead of 2 + G (more workgroups
 // per CU).  Same arithmetic, in the same order.
-template <int G, int PF, unsigned RM, int NT>
-__global__ void __launch_bounds__(NT)
+template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PF <= 6 ? 5 : 1)))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
     extern __shared__ float2 lds[];
@@ -793,7 +803,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
             }
         }
         __syncthreads();
-        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
         const int g = G == 1 ? 0 : (int)((grp_mask >> c) & 1);
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -814,7 +824,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
             if (off[u] >= 0) A[e] = cconj(acc[g][u]);
         }
         __syncthreads();
-        const float2* r = fft_lds_pp<RM>(A, Bf, fd, nt, nt, 1, true);
+        const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
         float2* dst = out + ((size_t)b * G + g) * n * (size_t)plane + j0;
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
@@ -1573,14 +1583,14 @@ static void launch_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec,
                        C, T, ntiles, grp_mask, lo, w);
 }
 
-template <int G, unsigned RM, int NT>
+template <int G, unsigned RM, int NT, int PF = kXcorrPF, class SP = DynamicTile>
 static void launch_reg_pf_nt(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                              float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
                              unsigned grp_mask, int lo, int w) {
     if (lds > 65536)
-        (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, kXcorrPF, RM, NT>,
+        (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, PF, RM, NT, SP>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL((k_xcorr_reg_pf<G, kXcorrPF, RM, NT>), grid, dim3(NT), lds, st, spec, tgt, out, fd, n, nhz,
+    hipLaunchKernelGGL((k_xcorr_reg_pf<G, PF, RM, NT, SP>), grid, dim3(NT), lds, st, spec, tgt, out, fd, n, nhz,
                        nb, C, T, ntiles, grp_mask, lo, w);
 }
 
@@ -1595,6 +1605,17 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
         launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     else if (n * T <= 128 * kXcorrPF)
         launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    else if (n * T <= 256 * 6) {
+        // compile-time tile plan: grid edge 175 = 5 * 5 * 7, 8 columns per
+        // tile, every tile full (175 * 88 columns = 1925 tiles of 8)
+        using S175 = StaticTile<8, 5, 5, 7>;
+        if (RM == kRadix57 && n == S175::kN && T == S175::kNL && (n * nhz) % T == 0 && static_plan_matches<5, 5, 7>(n))
+            launch_reg_pf_nt<G, RM, 256, 6, S175>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles,
+                                                  grp_mask, lo, w);
+        else
+            launch_reg_pf_nt<G, RM, 256, 6>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask,
+                                            lo, w);
+    }
     else
         launch_reg_pf_nt<G, RM, 256>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
 }
