@@ -390,6 +390,106 @@ struct StaticTile {
     }
 };
 
+// ---------------------------------------------------------------------------
+// Compile-time slab FFTs: the in-place, register-resident stages of fft_lds
+// (up to ceil(MAXV / R) butterflies per thread, read all, synchronise,
+// write all) with R, P, the span and the twiddle offset as constants, in two
+// layouts:
+//   tile  (TILE): element stride NL, line stride 1, consecutive threads on
+//                 consecutive lines (b = line + i NL);
+//   lines:        element stride 1, line stride N, consecutive threads on
+//                 consecutive butterflies of a line (b = i + line T).
+// ---------------------------------------------------------------------------
+
+template <int N, int NT, int MAXV, bool TILE, int NL, int R, int P, int TW>
+__device__ __forceinline__ void slab_stage_static(float2* buf, const float2* __restrict__ tw, int nlines) {
+    constexpr int T = N / R;
+    constexpr int MAXB = (MAXV + R - 1) / R;
+    constexpr int ES = TILE ? NL : 1;
+    const int nb = TILE ? T * NL : T * nlines;
+    float2 v[MAXB][R];
+    int obase[MAXB];
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int b = (int)threadIdx.x + m * NT;
+        obase[m] = -1;
+        if (b < nb) {
+            int i, ib;
+            if constexpr (TILE) {
+                i = b / NL;
+                ib = b;
+            } else {
+                const int line = b / T;
+                i = b - line * T;
+                ib = line * N + i;
+            }
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[m][j] = buf[ib + j * T * ES];
+            int k = 0;
+            if constexpr (P > 1) {
+                k = (int)((unsigned)i % (unsigned)P);
+                const float2* w = tw + TW + k;
+#pragma unroll
+                for (int j = 1; j < R; ++j) v[m][j] = cmul(v[m][j], w[(j - 1) * P]);
+            }
+            dft<R>(v[m]);
+            // (i - k) R + k = i + (R - 1)(i - k) elements along the line
+            obase[m] = ib + (R - 1) * ES * (i - k);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        if (obase[m] >= 0) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) buf[obase[m] + j * P * ES] = v[m][j];
+        }
+    }
+    __syncthreads();
+}
+
+template <int N, int NT, int MAXV, bool TILE, int NL, int P, int TW, int R, int... Rest>
+__device__ __forceinline__ void slab_fft_static_rec(float2* buf, const float2* tw, int nlines) {
+    slab_stage_static<N, NT, MAXV, TILE, NL, R, P, TW>(buf, tw, nlines);
+    if constexpr (sizeof...(Rest) > 0)
+        slab_fft_static_rec<N, NT, MAXV, TILE, NL, P * R, TW + (R - 1) * P, Rest...>(buf, tw, nlines);
+}
+
+template <int... Rs>
+struct StaticPlan {
+    static constexpr bool kStatic = true;
+    static constexpr int kN = (Rs * ...);
+    // the N/2 + 1 half-spectrum columns of a slab, tile layout
+    template <int NT, int MAXV>
+    __device__ __forceinline__ static void tile_inplace(float2* buf, const float2* tw) {
+        slab_fft_static_rec<kN, NT, MAXV, true, kN / 2 + 1, 1, 0, Rs...>(buf, tw, 0);
+    }
+    // nlines contiguous lines of length N
+    template <int NT, int MAXV>
+    __device__ __forceinline__ static void lines_inplace(float2* buf, const float2* tw, int nlines) {
+        slab_fft_static_rec<kN, NT, MAXV, false, 1, 1, 0, Rs...>(buf, tw, nlines);
+    }
+};
+
+struct DynamicPlan {
+    static constexpr bool kStatic = false;
+    static constexpr int kN = 0;
+};
+
+// fft_lds(buf, fd, nl, nl, 1, true) with nl = n / 2 + 1 columns
+template <class SP, int MAXV, unsigned RM, int NT>
+__device__ __forceinline__ void slab_fft_tile(float2* buf, const FftDesc& fd, int nl) {
+    if constexpr (SP::kStatic) SP::template tile_inplace<NT, MAXV>(buf, fd.twid);
+    else fft_lds<MAXV, RM>(buf, fd, nl, nl, 1, true);
+}
+
+// fft_lds(buf, fd, nlines, 1, n, false)
+template <class SP, int MAXV, unsigned RM, int NT>
+__device__ __forceinline__ void slab_fft_lines(float2* buf, const FftDesc& fd, int nlines) {
+    if constexpr (SP::kStatic) SP::template lines_inplace<NT, MAXV>(buf, fd.twid, nlines);
+    else fft_lds<MAXV, RM>(buf, fd, nlines, 1, fd.n, false);
+}
+
 // The runtime-descriptor form (any length and tile width).
 struct DynamicTile {
     static constexpr bool kStatic = false;

@@ -317,7 +317,7 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // ---------------------------------------------------------------------------
 
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
-template <int MAXV, int NT, unsigned RM = 0xffffffffu>
+template <int MAXV, int NT, unsigned RM = 0xffffffffu, class SP = DynamicPlan>
 __global__ void __launch_bounds__(NT)
 k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w) {
     // slabs x in [lo, lo + w) only; rows y outside [lo, lo + w) are zero, so
@@ -353,7 +353,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
         }
     }
     __syncthreads();
-    fft_lds<MAXV, RM>(lds + l0 * n, fd, l1 - l0, 1, n, false);
+    slab_fft_lines<SP, MAXV, RM, NT>(lds + l0 * n, fd, l1 - l0);
     // unpack Z = A + iB: A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
     {
         constexpr int MU = (MAXV + 1) / 2;
@@ -388,7 +388,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
         }
         __syncthreads();
     }
-    fft_lds<MAXV, RM>(lds, fd, nhz, nhz, 1, true);
+    slab_fft_tile<SP, MAXV, RM, NT>(lds, fd, nhz);
     float2* out = spec + (plane * n + x) * (size_t)n * nhz;
 #pragma unroll
     for (int m = 0; m < MAXV; ++m) {
@@ -1135,7 +1135,7 @@ int launch_xcorr_mfma(const float2* spec, const float2* tgt, float2* out, const 
 
 // Inverse: sum[bg][x] ([ky][kz]) -> real slab ([y][z]) -> argmax key.
 // Optionally writes the real correlation (prob-mode mixing).
-template <int MAXV, int NT, unsigned RM = 0xffffffffu>
+template <int MAXV, int NT, unsigned RM = 0xffffffffu, class SP = DynamicPlan>
 __global__ void __launch_bounds__(NT)
 k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
          unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
@@ -1161,7 +1161,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
         }
     }
     __syncthreads();
-    fft_lds<MAXV, RM>(lds, fd, nhz, nhz, 1, true);   // conj of the y-inverse
+    slab_fft_tile<SP, MAXV, RM, NT>(lds, fd, nhz);   // conj of the y-inverse
     const int npair = (n + 1) >> 1;
     const int half = n >> 1;
     const bool even = (n & 1) == 0;
@@ -1201,7 +1201,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
             if (dst[m] >= 0) lds[dst[m]] = Zc[m];
         __syncthreads();
     }
-    fft_lds<MAXV, RM>(lds, fd, npair, 1, n, false);
+    slab_fft_lines<SP, MAXV, RM, NT>(lds, fd, npair);
     // z[n] = conj(F): row 2l = F.x, row 2l+1 = -F.y
     float bv = -INFINITY;
     uint32_t bi = 0xffffffffu;
@@ -1508,6 +1508,17 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
         allow_lds(KER<MV, NTH, RMV>, lds);                                             \
         hipLaunchKernelGGL((KER<MV, NTH, RMV>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
     } while (0)
+#define VX_SLAB_LAUNCH_SP(KER, MV, NTH, RMV, SPT, ...)                                 \
+    do {                                                                               \
+        allow_lds(KER<MV, NTH, RMV, SPT>, lds);                                        \
+        hipLaunchKernelGGL((KER<MV, NTH, RMV, SPT>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
+    } while (0)
+
+// compile-time slab plan: grid edge 175 = 5 * 5 * 7
+using Slab175 = StaticPlan<5, 5, 7>;
+static bool use_slab175(const FftDesc& fd, int n) {
+    return fd.radix_mask == kRadix57 && n == Slab175::kN && static_plan_matches<5, 5, 7>(n);
+}
 
 int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
                   hipStream_t st, int nt_pref = 0) {
@@ -1518,6 +1529,8 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 17 && use_slab175(fd, n))
+        VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, kRadix57, Slab175, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_fwd_zy, 16, 1024, kRadix57, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
@@ -1537,6 +1550,8 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 17 && use_slab175(fd, n))
+        VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, kRadix57, Slab175, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_inv_yz, 16, 1024, kRadix57, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
