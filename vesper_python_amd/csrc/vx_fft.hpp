@@ -71,6 +71,15 @@ __device__ __forceinline__ int fdiv(int a, int d, float dinv) {
     return q;
 }
 
+// a / d for small non-negative a: an unsigned constant-divisor divide (a
+// multiply-shift) when the divisor is a compile-time constant (S), else the
+// float-reciprocal form above.
+template <bool S>
+__device__ __forceinline__ int qdiv(int a, int d, float dinv) {
+    if constexpr (S) return (int)((unsigned)a / (unsigned)d);
+    else return fdiv(a, d, dinv);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -319,9 +319,12 @@ __global__ void k_target_chan(const double* __restrict__ data, const float* __re
 // Forward: real slab q[plane][x] ([y][z]) -> spec[plane][x] ([ky][kz]).
 template <int MAXV, int NT, unsigned RM = 0xffffffffu, class SP = DynamicPlan>
 __global__ void __launch_bounds__(NT)
-k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w) {
+k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n_, int nhz_, int lo, int w) {
     // slabs x in [lo, lo + w) only; rows y outside [lo, lo + w) are zero, so
     // only the line pairs [l0, l1) covering them are transformed along z
+    // (a compile-time plan fixes the grid edge, and with it the index math)
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     extern __shared__ float2 lds[];
     const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
     const int x = lo + blockIdx.x;
@@ -341,7 +344,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
             re[m] = 0.f;
             im[m] = 0.f;
             if (e < nact) {
-                const int l = l0 + fdiv(e, n, ninv), z = e - (l - l0) * n;
+                const int l = l0 + qdiv<SP::kStatic>(e, n, ninv), z = e - (l - l0) * n;
                 re[m] = src[(size_t)(2 * l) * n + z];
                 if (2 * l + 1 < n) im[m] = src[(size_t)(2 * l + 1) * n + z];
             }
@@ -366,7 +369,7 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
             const int u = threadIdx.x + m * nthr;
             dst[m] = -1;
             if (u < nu) {
-                const int l = fdiv(u, nhz, nhzinv), k = u - l * nhz;
+                const int l = qdiv<SP::kStatic>(u, nhz, nhzinv), k = u - l * nhz;
                 A[m] = make_float2(0.f, 0.f);
                 B[m] = make_float2(0.f, 0.f);
                 if (l >= l0 && l < l1) {
@@ -751,7 +754,11 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PF <= 6 ? 5 : 1)))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
-               FftDesc fdg, int n, int nhz, int nb, int C, int T, int ntiles, unsigned grp_mask, int lo, int w) {
+               FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w) {
+    // a compile-time tile plan fixes the grid edge and the tile width
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
+    const int T = SP::kStatic ? SP::kNL : T_;
     extern __shared__ float2 lds[];
     int b, tile;
     if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
@@ -760,7 +767,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
     const int plane = n * nhz;
     const int j0 = tile * T;
-    const int nt = min(T, plane - j0);
+    const int nt = SP::kStatic ? T : min(T, plane - j0);   // static plans: every tile full
     const int ne = n * nt;
     const float ntinv = 1.0f / (float)nt;
     int off[PF];
@@ -769,7 +776,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
         const int e = threadIdx.x + u * NT;
         off[u] = -1;
         if (e < ne) {
-            const int xx = fdiv(e, nt, ntinv), t = e - xx * nt;
+            const int xx = qdiv<SP::kStatic>(e, nt, ntinv), t = e - xx * nt;
             off[u] = xx * plane + t;
             if (xx < lo || xx >= lo + w) off[u] |= 1 << 30;
         }
@@ -1137,8 +1144,10 @@ int launch_xcorr_mfma(const float2* spec, const float2* tgt, float2* out, const 
 // Optionally writes the real correlation (prob-mode mixing).
 template <int MAXV, int NT, unsigned RM = 0xffffffffu, class SP = DynamicPlan>
 __global__ void __launch_bounds__(NT)
-k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
+k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
          unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     extern __shared__ float2 lds[];
     const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
     __shared__ float s_v[16];
@@ -1179,7 +1188,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
             const int u = threadIdx.x + m * nthr;
             dst[m] = -1;
             if (u < nu) {
-                const int l = fdiv(u, n, ninv_p), k = u - l * n;
+                const int l = qdiv<SP::kStatic>(u, n, ninv_p), k = u - l * n;
                 const bool hasb = (2 * l + 1) < n;
                 if (k <= half) {
                     float2 ra = lds[(2 * l) * nhz + k];
@@ -1207,7 +1216,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G,
     uint32_t bi = 0xffffffffu;
     const size_t n3 = (size_t)n * n * n;
     for (int e = threadIdx.x; e < npair * n; e += nthr) {
-        const int l = fdiv(e, n, ninv_p), z = e - l * n;
+        const int l = qdiv<SP::kStatic>(e, n, ninv_p), z = e - l * n;
         const float2 f = lds[e];
         const uint32_t ia = ((uint32_t)x * n + 2 * l) * n + z;
         const float va = f.x;
