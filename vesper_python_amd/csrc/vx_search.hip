@@ -752,7 +752,7 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 // workgroup needs two tile buffers of LDS instead of 2 + G (more workgroups
 // per CU).  Same arithmetic, in the same order.
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PF <= 6 ? 5 : 1)))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G == 1 && PF <= 6 ? 5 : 1)))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w) {
     // a compile-time tile plan fixes the grid edge and the tile width
@@ -1631,7 +1631,9 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
         launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     else if (n * T <= 256 * 6) {
         // compile-time tile plan: grid edge 175 = 5 * 5 * 7, 8 columns per
-        // tile, every tile full (175 * 88 columns = 1925 tiles of 8)
+        // tile, every tile full (175 * 88 columns = 1925 tiles of 8).  (7
+        // columns, which fit every stage in one pass of 256 threads, measured
+        // 30 % slower: 56-byte column runs split the 128-byte lines.)
         using S175 = StaticTile<8, 5, 5, 7>;
         if (RM == kRadix57 && n == S175::kN && T == S175::kNL && (n * nhz) % T == 0 && static_plan_matches<5, 5, 7>(n))
             launch_reg_pf_nt<G, RM, 256, 6, S175>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles,
