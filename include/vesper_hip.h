@@ -112,6 +112,10 @@ int vx_plan_destroy(vx_plan* plan);
 #define VX_TUNE_STREAMS 7 /* 2: two batches in flight on two workspaces and streams (the
                              second stream is forked from and joined to the caller's);
                              1 = one (default).  Identical results. */
+#define VX_TUNE_STATIC_PLANS 8 /* 1 (default): grid edges with compile-time FFT stage plans
+                                  (175, 128) run kernels with every stage's radix, span and
+                                  twiddle offset as constants; 0 = runtime descriptors only.
+                                  Identical results. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

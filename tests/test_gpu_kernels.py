@@ -455,6 +455,47 @@ def test_slab_shapes_identical_at_n175(gpu):
     _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=175")
 
 
+@pytest.mark.parametrize("n,mode", [(175, nat.VX_MODE_VEC), (175, nat.VX_MODE_PROB),
+                                    (128, nat.VX_MODE_VEC), (128, nat.VX_MODE_CC)])
+def test_static_plans_identical(gpu, n, mode):
+    """Grid edges with compile-time stage plans (175 = 5 5 7, 128 = 8 8 2):
+    the slab and x-pass kernels built on them give bit-identical scores and
+    translations to the runtime-descriptor kernels, in the one-group modes
+    and in prob mode (two groups, mixed score); at 128 the rows also match
+    the FP64 oracle."""
+    rng = np.random.default_rng(5700 + n + mode)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    inner = np.linalg.norm(g, axis=-1) <= 0.2 * n
+    td, tv = _maps(rng, n, 0.03)
+    sd, sv = _maps(rng, n, 0.3)
+    sd[~inner] = 0
+    sv[~inner] = 0
+    angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 170], [10, 350, 45]], np.float64)
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    prob = None
+    if mode == nat.VX_MODE_PROB:
+        tp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (td != 0)
+        sp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (sd != 0)
+        prob = (_dev(tp, torch.float32), _dev(sp, torch.float32))
+    res = []
+    for static in (True, False):
+        _say("static plans", n, mode, static)
+        plan = engine.SearchPlan(n, mode, batch=2)
+        plan.tune(static_plans=static, xpass_variant=3, xpass_tile=8)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), prob[0] if prob else None, 0.3)
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), prob[1] if prob else None, 0.3)
+        if prob:
+            plan.set_prob_stats(0.4, 2.0, 0.1, 3.0, 0.2, enable=True)
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    if n == 128 and mode == nat.VX_MODE_VEC:
+        o32, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref32")
+        o64, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref64")
+        _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=128")
+
+
 @pytest.mark.parametrize("n,mode", [(35, nat.VX_MODE_VEC), (24, nat.VX_MODE_CC)])
 def test_xpass_variants_identical(gpu, n, mode):
     """The FFT x-pass kernels (register, LDS, register-prefetch; several

@@ -22,6 +22,7 @@ VX_TUNE_PIN_BOX_WIDTH = 4
 VX_TUNE_PIN_BOX_LO = 5
 VX_TUNE_SLAB_THREADS = 6
 VX_TUNE_STREAMS = 7
+VX_TUNE_STATIC_PLANS = 8
 ABI_VERSION = 1
 
 MODE_CODES = {"VecProduct": VX_MODE_VEC, "Overlap": VX_MODE_OVERLAP, "CC": VX_MODE_CC,

@@ -464,10 +464,24 @@ __device__ __forceinline__ void slab_fft_static_rec(float2* buf, const float2* t
         slab_fft_static_rec<N, NT, MAXV, TILE, NL, P * R, TW + (R - 1) * P, Rest...>(buf, tw, nlines);
 }
 
+template <int NL, int... Rs>
+struct StaticTile;
+
 template <int... Rs>
 struct StaticPlan {
     static constexpr bool kStatic = true;
     static constexpr int kN = (Rs * ...);
+    static constexpr unsigned kMask = (radix_bit(Rs) | ...);
+    // the same radices as a column-tile plan of NL lines
+    template <int NL>
+    using Tile = StaticTile<NL, Rs...>;
+    __host__ static bool same_radices(const int* r, int nr) {
+        constexpr int want[] = {Rs...};
+        if (nr != (int)sizeof...(Rs)) return false;
+        for (int i = 0; i < nr; ++i)
+            if (r[i] != want[i]) return false;
+        return true;
+    }
     // the N/2 + 1 half-spectrum columns of a slab, tile layout
     template <int NT, int MAXV>
     __device__ __forceinline__ static void tile_inplace(float2* buf, const float2* tw) {

@@ -47,14 +47,28 @@ static bool factorize(int n, std::vector<int>& r) {
     return m == 1 && (int)r.size() <= kMaxStages;
 }
 
-// True when fft_desc_create's stage order for n is exactly Rs... (the
+// True when fft_desc_create's stage order for n is exactly SP's radices (the
 // compile-time plan then reads the descriptor's twiddle table unchanged).
-template <int... Rs>
-static bool static_plan_matches(int n) {
+template <class SP>
+static bool plan_matches(int n) {
     std::vector<int> r;
-    if (!factorize(n, r)) return false;
-    const std::vector<int> want{Rs...};
-    return r == want;
+    return n == SP::kN && factorize(n, r) && SP::same_radices(r.data(), (int)r.size());
+}
+
+// Grid edges with compile-time stage plans (radices in fft_desc_create's
+// order): kernels instantiated for these lengths carry every stage's radix,
+// span and twiddle offset as constants.  175 = 5 * 5 * 7 is the grid edge of
+// a 200^3 map at 2 A with the reference's padding; 128 = 8 * 8 * 2.
+template <class... Ps>
+struct PlanList {};
+using StaticLengths = PlanList<StaticPlan<5, 5, 7>, StaticPlan<8, 8, 2>>;
+
+// Calls f(SP{}) for the listed plan whose length is n; false if none is.
+template <class F, class... Ps>
+static bool with_static_plan(int n, PlanList<Ps...>, F&& f) {
+    bool hit = false;
+    ((hit = hit || (plan_matches<Ps>(n) && (f(Ps{}), true))), ...);
+    return hit;
 }
 
 int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
@@ -1523,14 +1537,8 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
         hipLaunchKernelGGL((KER<MV, NTH, RMV, SPT>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
     } while (0)
 
-// compile-time slab plan: grid edge 175 = 5 * 5 * 7
-using Slab175 = StaticPlan<5, 5, 7>;
-static bool use_slab175(const FftDesc& fd, int n) {
-    return fd.radix_mask == kRadix57 && n == Slab175::kN && static_plan_matches<5, 5, 7>(n);
-}
-
 int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
-                  hipStream_t st, int nt_pref = 0) {
+                  hipStream_t st, int nt_pref = 0, bool use_static = true) {
     const int nhz = n / 2 + 1;
     int nthr, maxv;
     slab_shape(n, nhz, &nthr, &maxv, nt_pref);
@@ -1538,9 +1546,11 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
-    else if (maxv == 17 && use_slab175(fd, n))
-        VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, kRadix57, Slab175, q, spec, fd, n, nhz, lo, w);
-    else if (maxv == 17 && fd.radix_mask == kRadix57)
+    else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
+                 using SP = decltype(sp);
+                 VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+             })) {
+    } else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_fwd_zy, 16, 1024, kRadix57, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
@@ -1551,7 +1561,8 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
 }
 
 int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
-                  unsigned long long* keys, int nkeys, float* real_out, hipStream_t st, int nt_pref = 0) {
+                  unsigned long long* keys, int nkeys, float* real_out, hipStream_t st, int nt_pref = 0,
+                  bool use_static = true) {
     const int nhz = n / 2 + 1;
     int nthr, maxv;
     slab_shape(n, nhz, &nthr, &maxv, nt_pref);
@@ -1559,9 +1570,11 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
     if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
-    else if (maxv == 17 && use_slab175(fd, n))
-        VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, kRadix57, Slab175, sum, fd, n, nhz, G, keys, nkeys, real_out);
-    else if (maxv == 17 && fd.radix_mask == kRadix57)
+    else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
+                 using SP = decltype(sp);
+                 VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+             })) {
+    } else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_inv_yz, 16, 1024, kRadix57, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -1624,31 +1637,35 @@ static void launch_reg_pf_nt(dim3 grid, size_t lds, hipStream_t st, const float2
 template <int G, unsigned RM>
 static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                           float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
-                          unsigned grp_mask, int lo, int w) {
+                          unsigned grp_mask, int lo, int w, bool use_static) {
+    // compile-time tile plans: 8 columns per tile, every tile full (the
+    // grid edges listed have n (n / 2 + 1) divisible by 8), with the same
+    // threads and values per thread as the runtime form below
+    if (use_static && T == 8 && (n * nhz) % 8 == 0 &&
+        with_static_plan(n, StaticLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            constexpr int E = SP::kN * 8;
+            constexpr int NT = E <= 64 * kXcorrPF ? 64 : (E <= 128 * kXcorrPF ? 128 : 256);
+            constexpr int PF = (NT == 256 && E <= 256 * 6) ? 6 : kXcorrPF;
+            static_assert(E <= NT * PF, "tile exceeds the workgroup's values");
+            launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(grid, lds, st, spec, tgt, out, fd,
+                                                                                 n, nhz, nb, C, T, ntiles, grp_mask,
+                                                                                 lo, w);
+        }))
+        return;
     if (n * T <= 64 * kXcorrPF)
         launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     else if (n * T <= 128 * kXcorrPF)
         launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-    else if (n * T <= 256 * 6) {
-        // compile-time tile plan: grid edge 175 = 5 * 5 * 7, 8 columns per
-        // tile, every tile full (175 * 88 columns = 1925 tiles of 8).  (7
-        // columns, which fit every stage in one pass of 256 threads, measured
-        // 30 % slower: 56-byte column runs split the 128-byte lines.)
-        using S175 = StaticTile<8, 5, 5, 7>;
-        if (RM == kRadix57 && n == S175::kN && T == S175::kNL && (n * nhz) % T == 0 && static_plan_matches<5, 5, 7>(n))
-            launch_reg_pf_nt<G, RM, 256, 6, S175>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles,
-                                                  grp_mask, lo, w);
-        else
-            launch_reg_pf_nt<G, RM, 256, 6>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask,
-                                            lo, w);
-    }
+    else if (n * T <= 256 * 6)
+        launch_reg_pf_nt<G, RM, 256, 6>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
     else
         launch_reg_pf_nt<G, RM, 256>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
                  int nb, int C, int G, unsigned grp_mask, int lo, int w, hipStream_t st,
-                 int xvariant = 0, int xT = 0) {
+                 int xvariant = 0, int xT = 0, bool use_static = true) {
     const int nhz = n / 2 + 1;
     if (tgt != nullptr && xvariant == 3) {
         const int T = xT > 0 ? xT : 8;
@@ -1660,11 +1677,11 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
         dim3 grid((unsigned)(((ntiles + 7) / 8) * 8 * nb));
         const unsigned rm = radix_mask(fd);
         if (G == 1) {
-            if (rm == kRadix57) launch_reg_pf<1, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-            else                launch_reg_pf<1, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            if (rm == kRadix57) launch_reg_pf<1, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w, use_static);
+            else                launch_reg_pf<1, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w, use_static);
         } else {
-            if (rm == kRadix57) launch_reg_pf<2, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-            else                launch_reg_pf<2, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+            if (rm == kRadix57) launch_reg_pf<2, kRadix57>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w, use_static);
+            else                launch_reg_pf<2, kAllRadices>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w, use_static);
         }
         VX_LAUNCH_CHECK();
         return VX_OK;
@@ -1772,6 +1789,7 @@ struct vx_plan {
     int pin_lo;
     int slab_nt;      // preferred threads per slab workgroup (0 = default)
     int nstreams;     // 2: alternate batches over two workspaces and streams
+    int static_plans; // 1: compile-time stage plans for the grid edges that have one
     float* q2;        // second workspace (nstreams == 2)
     double* qd2;
     float2* spec2;
@@ -1829,6 +1847,7 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     p->xvariant = n * 8 <= 256 * kXcorrPF ? 3 : 1;
     p->xT = 0;
     p->nstreams = 1;
+    p->static_plans = 1;
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
@@ -2004,7 +2023,7 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
     // spec holds B*C planes >= C planes.
     int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st)
-                       : launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st);
+                       : launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st, 0, p->static_plans != 0);
     if (rc != VX_OK) return rc;
     // treat the C channels as "batch" 1 with C channels, forward-conj mode
     rc = launch_xcorr(p->spec, nullptr, p->tgt, p->fd, n, 1, p->C, 1, 0u, 0, n, st);
@@ -2084,6 +2103,10 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
             }
             p->nstreams = value;
             break;
+        case VX_TUNE_STATIC_PLANS:
+            VX_REQUIRE(value == 0 || value == 1, VX_ERR_ARG, "static plans must be 0 or 1");
+            p->static_plans = value;
+            break;
         case VX_TUNE_SLAB_THREADS:
             VX_REQUIRE(value == 0 || value == 512 || value == 1024, VX_ERR_ARG,
                        "slab threads must be 0, 512 or 1024");
@@ -2148,7 +2171,8 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     }
     VX_MARK(1);
     rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st)
-                   : launch_fwd_zy(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt);
+                   : launch_fwd_zy(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt,
+                                   p->static_plans != 0);
     if (rc != VX_OK) return rc;
     VX_MARK(2);
     if (p->xvariant == 2) {
@@ -2159,13 +2183,14 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
         rc = launch_xcorr_mfma(ws.spec, p->tgt, ws.sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, p->xT, st);
     } else {
         rc = launch_xcorr(ws.spec, p->tgt, ws.sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
-                          p->xvariant, p->xT);
+                          p->xvariant, p->xT, p->static_plans != 0);
     }
     if (rc != VX_OK) return rc;
     VX_MARK(3);
     float* ro = (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
     rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st)
-                   : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt);
+                   : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt,
+                                   p->static_plans != 0);
     if (rc != VX_OK) return rc;
     VX_MARK(4);
     if (p->mode == VX_MODE_PROB && p->mix) {
@@ -2255,12 +2280,14 @@ extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float*
     for (int c = 0; c < nch; ++c) {
         const float* qc = q + (size_t)c * n3;
         int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st)
-                           : launch_fwd_zy(qc, p->spec, p->fd, n, 1, 0, n, st);
+                           : launch_fwd_zy(qc, p->spec, p->fd, n, 1, 0, n, st, 0, p->static_plans != 0);
         if (rc != VX_OK) return rc;
-        rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT);
+        rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT,
+                          p->static_plans != 0);
         if (rc != VX_OK) return rc;
         rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st)
-                       : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st);
+                       : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, 0,
+                                       p->static_plans != 0);
         if (rc != VX_OK) return rc;
     }
     // the q planes were not touched, but the keys were: a later search re-zeroes them per batch
