@@ -359,8 +359,10 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
             im[m] = 0.f;
             if (e < nact) {
                 const int l = l0 + qdiv<SP::kStatic>(e, n, ninv), z = e - (l - l0) * n;
-                re[m] = src[(size_t)(2 * l) * n + z];
-                if (2 * l + 1 < n) im[m] = src[(size_t)(2 * l + 1) * n + z];
+                if (z >= lo && z < lo + w) {   // the planes are zero outside the box: no load
+                    re[m] = src[(size_t)(2 * l) * n + z];
+                    if (2 * l + 1 < n) im[m] = src[(size_t)(2 * l + 1) * n + z];
+                }
             }
         }
 #pragma unroll
