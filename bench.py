@@ -318,13 +318,17 @@ def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
     FFT-score 5 % of the -A `ang` rows in two rounds with retraining; recall
     of the exhaustive top-10 (`best` holds the exhaustive scores) and
     rotation-equivalents per second (R rows covered / wall time of the whole
-    proposal pipeline, training included)."""
+    proposal pipeline, training included, after one untimed warm-up run)."""
     import torch
     from vesper_python_amd import engine, proposer
     from vesper_python_amd.geometry import calc_angle_comb
     truth = engine.topn(best[:, 0].contiguous(), 10)
     fine = calc_angle_comb(ang)
     coarse = calc_angle_comb(30.0)
+    # one untimed run first, as the exhaustive search gets its warm-up step:
+    # the first call loads the network's kernels and grows the allocator
+    pr = proposer.Proposer(plan, *n1._dev, *n2._dev, proposer.RotationMLP(seed=7))
+    proposer.propose_rounds(pr, coarse, fine, frac=0.05, rounds=2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     pr = proposer.Proposer(plan, *n1._dev, *n2._dev, proposer.RotationMLP(seed=7))

@@ -149,19 +149,19 @@ class RotationMLP:
         bp = pad64(min(batch, M))
         xb = torch.zeros((bp, IN), dtype=torch.bfloat16, device=self.dev)
         yb = torch.zeros(bp, dtype=torch.float32, device=self.dev)
-        losses = []
-        for _ in range(epochs):
+        # per-epoch loss sums stay on the device until the end (no host sync
+        # per epoch)
+        tot = torch.zeros(epochs, device=self.dev)
+        for e in range(epochs):
             perm = torch.from_numpy(rng.permutation(M)).to(self.dev)
-            tot = torch.zeros(1, device=self.dev)
             for s in range(0, M, batch):
                 idx = perm[s:s + batch]
                 m = int(idx.numel())
                 xb[:m] = X.index_select(0, idx)
                 xb[m:] = 0
                 yb[:m] = y.index_select(0, idx)
-                tot += self.train_step(xb, yb, m, lr) * m
-            losses.append(float(tot.item()) / M)
-        return losses
+                tot[e:e + 1] += self.train_step(xb, yb, m, lr) * m
+        return [float(v) / M for v in tot.cpu().numpy()]
 
     def state(self) -> dict:
         return {f"p{i}": p.cpu().numpy() for i, p in enumerate(self.params)}
