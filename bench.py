@@ -42,15 +42,15 @@ def _stage_bytes(n: int, w: int, b: int) -> dict:
     each stage, with the query's support box [lo, lo + w)^3:
       gather   reads the float4 source under the box once, writes 3 float32
                planes inside the box;
-      fwd_zy   reads the w slabs' rows inside the box, writes 3 half spectra
-               of the w slabs;
+      fwd_zy   reads the w slabs' rows inside the box (z inside the box
+               too), writes 3 half spectra of the w slabs;
       xcorr    reads those spectra, the 3 target spectra once per launch,
                writes 1 half spectrum per rotation;
       inv_yz   reads that spectrum."""
     nhz = n // 2 + 1
     nh = n * n * nhz
     return {"gather": b * (16 + 12) * w ** 3,
-            "fwd_zy": b * 3 * (w * (w + 1) * n * 4 + w * n * nhz * 8),
+            "fwd_zy": b * 3 * (w * (w + 1) * w * 4 + w * n * nhz * 8),
             "xcorr": b * (3 * w * n * nhz * 8 + nh * 8) + 3 * nh * 8,
             "inv_yz": b * nh * 8}
 
