@@ -764,17 +764,22 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 }
 
 // Register-accumulating form of k_xcorr_lds_pf: each thread owns the same
-// tile elements in every phase, so the group sums live in registers and the
+// tile elements in every phase, so the group sum lives in registers and the
 // workgroup needs two tile buffers of LDS instead of 2 + G (more workgroups
-// per CU).  Same arithmetic, in the same order.
+// per CU).  The groups run one after the other (a group's channels in
+// channel order, then its inverse), so one accumulator set serves every
+// group: the same arithmetic in the same order as accumulating all groups
+// at once.
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(G == 1 && PF <= 6 ? 5 : 1)))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PF <= 6 ? 5 : 1)))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w) {
     // a compile-time tile plan fixes the grid edge and the tile width
     const int n = SP::kStatic ? SP::kN : n_;
     const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     const int T = SP::kStatic ? SP::kNL : T_;
+    // static plans: every tile is full, so slots below kFull always hold an element
+    constexpr int kFull = SP::kStatic ? SP::kN * SP::kNL : 0;
     extern __shared__ float2 lds[];
     int b, tile;
     if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
@@ -783,7 +788,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
     const int plane = n * nhz;
     const int j0 = tile * T;
-    const int nt = SP::kStatic ? T : min(T, plane - j0);   // static plans: every tile full
+    const int nt = SP::kStatic ? T : min(T, plane - j0);
     const int ne = n * nt;
     const float ntinv = 1.0f / (float)nt;
     int off[PF];
@@ -791,60 +796,72 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     for (int u = 0; u < PF; ++u) {
         const int e = threadIdx.x + u * NT;
         off[u] = -1;
-        if (e < ne) {
+        if ((u + 1) * NT <= kFull || e < ne) {
             const int xx = qdiv<SP::kStatic>(e, nt, ntinv), t = e - xx * nt;
             off[u] = xx * plane + t;
             if (xx < lo || xx >= lo + w) off[u] |= 1 << 30;
         }
     }
-    float2 acc[G][PF];
-#pragma unroll
-    for (int g = 0; g < G; ++g)
-#pragma unroll
-        for (int u = 0; u < PF; ++u) acc[g][u] = make_float2(0.f, 0.f);
+#define VX_SLOT(u) ((u + 1) * NT <= kFull || off[u] >= 0)
+    // channel order: group 0's channels, then group 1's (G == 2)
+    auto in_group = [&](int c, int g) { return G == 1 || (int)((grp_mask >> c) & 1) == g; };
+    auto next_channel = [&](int c, int g) {   // next channel of group g after c, else -1
+        for (int k = c + 1; k < C; ++k)
+            if (in_group(k, g)) return k;
+        return -1;
+    };
     const float2* sb = spec + (size_t)b * C * n * (size_t)plane + j0;
-    float2 sv[PF], tv[PF];
-#pragma unroll
-    for (int u = 0; u < PF; ++u) {
-        sv[u] = make_float2(0.f, 0.f);
-        if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sb[off[u]];
-    }
-    for (int c = 0; c < C; ++c) {
-#pragma unroll
-        for (int u = 0; u < PF; ++u)
-            if (off[u] >= 0) A[threadIdx.x + u * NT] = sv[u];
-        const float2* tc = tgt + (size_t)c * n * plane + j0;
-#pragma unroll
-        for (int u = 0; u < PF; ++u)
-            if (off[u] >= 0) tv[u] = tc[off[u] & ~(1 << 30)];
-        if (c + 1 < C) {
-            const float2* sn = sb + (size_t)(c + 1) * n * plane;
-#pragma unroll
-            for (int u = 0; u < PF; ++u) {
-                sv[u] = make_float2(0.f, 0.f);
-                if (off[u] >= 0 && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
-            }
-        }
-        __syncthreads();
-        const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
-        const int g = G == 1 ? 0 : (int)((grp_mask >> c) & 1);
+    float2 sv[PF], tv[PF], acc[PF];
+    int c = next_channel(-1, 0);
+    int gc = 0;   // group of the prefetched channel
+    if (c < 0 && G == 2) { c = next_channel(-1, 1); gc = 1; }
+    if (c >= 0) {
+        const float2* sn = sb + (size_t)c * n * plane;
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
-            const int e = threadIdx.x + u * NT;
-            if (off[u] >= 0) {
-                const float2 pr = cmul(r[e], tv[u]);
-                if (G == 1 || g == 0) acc[0][u] = cadd(acc[0][u], pr);
-                else                  acc[G - 1][u] = cadd(acc[G - 1][u], pr);
-            }
+            sv[u] = make_float2(0.f, 0.f);
+            if (VX_SLOT(u) && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
         }
-        __syncthreads();
     }
-#pragma unroll
+#pragma unroll 1
     for (int g = 0; g < G; ++g) {
 #pragma unroll
+        for (int u = 0; u < PF; ++u) acc[u] = make_float2(0.f, 0.f);
+        for (; c >= 0 && gc == g;) {
+            // the channel after this one: the next of this group, else the
+            // first of the next group
+            int pc = next_channel(c, g), pg = g;
+            if (pc < 0 && g + 1 < G) { pc = next_channel(-1, g + 1); pg = g + 1; }
+#pragma unroll
+            for (int u = 0; u < PF; ++u)
+                if (VX_SLOT(u)) A[threadIdx.x + u * NT] = sv[u];
+            const float2* tc = tgt + (size_t)c * n * plane + j0;
+#pragma unroll
+            for (int u = 0; u < PF; ++u)
+                if (VX_SLOT(u)) tv[u] = tc[off[u] & ~(1 << 30)];
+            if (pc >= 0) {
+                const float2* sn = sb + (size_t)pc * n * plane;
+#pragma unroll
+                for (int u = 0; u < PF; ++u) {
+                    sv[u] = make_float2(0.f, 0.f);
+                    if (VX_SLOT(u) && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
+                }
+            }
+            __syncthreads();
+            const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
+#pragma unroll
+            for (int u = 0; u < PF; ++u) {
+                const int e = threadIdx.x + u * NT;
+                if (VX_SLOT(u)) acc[u] = cadd(acc[u], cmul(r[e], tv[u]));
+            }
+            __syncthreads();
+            c = pc;
+            gc = pg;
+        }
+#pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int e = threadIdx.x + u * NT;
-            if (off[u] >= 0) A[e] = cconj(acc[g][u]);
+            if (VX_SLOT(u)) A[e] = cconj(acc[u]);
         }
         __syncthreads();
         const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
@@ -852,10 +869,11 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int e = threadIdx.x + u * NT;
-            if (off[u] >= 0) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
+            if (VX_SLOT(u)) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
         }
         __syncthreads();
     }
+#undef VX_SLOT
 }
 
 // ---------------------------------------------------------------------------
