@@ -269,6 +269,50 @@ def measure_library_fft(n1, n2, rot, batch, plan):
             "what": "torch.fft (rocFFT) r2c x 3 channels, multiply, sum, c2r, max; rotation excluded"}
 
 
+def measure_uneven_pairs(npairs: int = 6, ang: float = 30.0, s: float = 1.5, g: float = 16.0,
+                         seed0: int = 4000):
+    """Many pairs of uneven size in one call of `batch.search_pairs`: pairs
+    from the uneven generator (A edge U{96..320} voxels at 1 A, 40-60 %
+    sub-chains), resampled at -s 1.5 and fastVEC'd, then every -A `ang` row of
+    every pair scored with one plan and one captured graph per grid edge,
+    replayed per pair.  Rows/s of the whole call (plan setup and capture
+    included), after one untimed call; the eager call (no graphs) beside it,
+    and whether both give the same top rows."""
+    import torch
+    from vesper_python_amd import batch, synth
+    from vesper_python_amd import VESPER_1_prob_new as V
+    from vesper_python_amd import main as cli
+    from vesper_python_amd.geometry import calc_angle_comb
+    tmp = tempfile.mkdtemp(prefix="vx_pairs_")
+    pairs = []
+    for k in range(npairs):
+        pair = synth.make_uneven_pair(seed0 + k)
+        paths = synth.write_pair(pair, os.path.join(tmp, f"p{k}"))
+        with contextlib.redirect_stdout(io.StringIO()):
+            m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], pair.a.threshold, pair.b.threshold, s, g)
+            n1 = V.fastVEC(m1, n1, g)
+            n2 = V.fastVEC(m2, n2, g)
+        pairs.append((n1, n2))
+    rows = len(calc_angle_comb(ang)) * npairs
+    res = {}
+    for use_graph in (True, False):
+        batch.search_pairs(pairs, ang=ang, use_graph=use_graph)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = batch.search_pairs(pairs, ang=ang, use_graph=use_graph)
+        torch.cuda.synchronize()
+        res[use_graph] = (time.perf_counter() - t0, out)
+    same = all(np.array_equal(a["top"], b["top"]) for a, b in zip(res[True][1], res[False][1]))
+    edges = [int(p[0].xdim) for p in pairs]
+    return {"pairs": npairs, "grid_edges": edges, "graphed_pairs": sum(edges.count(e) > 1 for e in edges),
+            "rows": rows,
+            "rows_per_s_graph": round(rows / res[True][0], 1), "rows_per_s_eager": round(rows / res[False][0], 1),
+            "same_top_rows": bool(same),
+            "what": f"uneven pairs (A edge U{{96..320}}), -s {s} -g {g} -A {ang}; one plan per grid edge, a "
+                    "captured graph replayed per pair where pairs share an edge, plan setup and capture inside "
+                    "the timed call"}
+
+
 def measure_interp(edge: int = 337, cpu: bool = True):
     """Prediction-grid interpolation (interpolation_functions.py:6-135) on a
     synthetic [edge]^3 x 4 float32 grid with values on one parity lattice,
@@ -361,6 +405,7 @@ def main():
     ap.add_argument("--no-prob", action="store_true", help="skip the prob-mode measurement")
     ap.add_argument("--no-interp", action="store_true", help="skip the prediction-grid interpolation measurement")
     ap.add_argument("--no-libfft", action="store_true", help="skip the vendor-FFT pipeline reference point")
+    ap.add_argument("--no-pairs", action="store_true", help="skip the uneven multi-pair measurement")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -475,6 +520,10 @@ def main():
     if not args.no_libfft and not args.rows:
         libfft = measure_library_fft(n1, n2, rot, args.batch, plan)
 
+    pairs_m = None
+    if not args.no_pairs and not args.rows:
+        pairs_m = measure_uneven_pairs()
+
     interp = None
     if not args.no_interp and not args.rows:
         interp = measure_interp(cpu=not args.no_cpu)
@@ -505,7 +554,7 @@ def main():
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
-           "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft}
+           "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     if cpu_all:

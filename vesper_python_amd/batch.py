@@ -7,8 +7,8 @@ rotation rows and the outputs keep their addresses, so a new pair only
 rewrites the plan's target spectra and query source before the replay.  The
 support box is pinned to the largest one of the group so that every pair has
 the same launch shapes (the union of the group's boxes).  Pairs of uneven size are grouped by grid edge; each
-group gets a batch size that keeps one launch's work near that of 8 rotations
-at N = 175.
+group gets a batch size that keeps one launch's work near that of 32 rotations
+at N = 175.  A group of one pair runs eagerly.
 """
 from __future__ import annotations
 
@@ -91,15 +91,22 @@ def search_pairs(pairs, ang=10.0, mode="VecProduct", topn=10, batch=None, use_gr
             hi = max(hi, info["box_lo"] + info["box_width"] - 1)
         plan.pin_box(lo, hi - lo + 1)
         gs = None
+        # a group of one pair runs eagerly: capturing costs one extra step
+        # (the warm-up before capture) and there would be nothing to replay
+        graph = use_graph and len(idxs) > 1
         for i in idxs:
             t, s = pairs[i]
             ave = t.ave if t.ave is not None else 0.0
             plan.set_target(*dm[i][0], None, ave)
             plan.set_search(*dm[i][1], None, ave)
-            if use_graph:
+            if graph:
                 if gs is None:
+                    # the warm-up step before capture scores this pair (capture
+                    # records without running), so its results are taken as is
                     gs = GraphedSearch(plan, rot, topn)
-                best, trans, rows = gs.replay()
+                    best, trans, rows = gs.best, gs.trans, gs.rows
+                else:
+                    best, trans, rows = gs.replay()
                 top = rows[:gs.k].cpu().numpy()
             else:
                 best, trans = plan.score_rows(rot)
