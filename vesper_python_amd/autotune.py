@@ -103,7 +103,12 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
     # slab workgroup shape or separate z and y passes, on top of the best x
     # pass (the stages are independent); the pencil passes round differently,
     # so they are held to the same 1e-4 gate as the MFMA x pass
-    for cand in (dict(best, slab_threads=512), dict(best, pencil=True)):
+    try:
+        plan.tune(pencil=False)
+        slab_ok = True
+    except Exception:
+        slab_ok = False   # the slab exceeds LDS: the pencil passes are the only form
+    for cand in ((dict(best, slab_threads=512), dict(best, pencil=True)) if slab_ok else ()):
         try:
             plan.tune(**cand)
             plan.score_rows(sample[:plan.batch])
