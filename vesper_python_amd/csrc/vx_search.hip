@@ -1717,7 +1717,9 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
         return VX_OK;
     }
     if (xvariant == 1) {
-        int T = xT > 0 ? xT : 8;
+        // default tile: 8 columns, 4 above grid edge 256 (the (2 + G) tile
+        // buffers then stay near 32 KB, four workgroups per CU; measured at 343)
+        int T = xT > 0 ? xT : (n <= 256 ? 8 : 4);
         while (T > 1 && ((size_t)(2 + G) * n * T + fd.ntw) * sizeof(float2) > kMaxLds) T >>= 1;
         const size_t lds = ((size_t)(2 + G) * n * T + fd.ntw) * sizeof(float2);
         VX_REQUIRE(lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large for the x pass", n);
