@@ -1405,9 +1405,10 @@ static int pencil_pairs(int n) {
 }
 
 static int pencil_T(int n) {
-    int T = 4096 / n;
-    if (T < 1) T = 1;
-    if (T > 16) T = 16;
+    // columns per y-pass workgroup: a power of two (runs of 8 T bytes stay
+    // aligned to 64-byte segments), ~4k complex values, at most 16
+    int T = 16;
+    while (T > 1 && T * n > 4096) T >>= 1;
     return T;
 }
 
