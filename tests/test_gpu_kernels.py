@@ -456,13 +456,18 @@ def test_slab_shapes_identical_at_n175(gpu):
 
 
 @pytest.mark.parametrize("n,mode", [(175, nat.VX_MODE_VEC), (175, nat.VX_MODE_PROB),
-                                    (128, nat.VX_MODE_VEC), (128, nat.VX_MODE_CC)])
+                                    (128, nat.VX_MODE_VEC), (128, nat.VX_MODE_CC),
+                                    (96, nat.VX_MODE_VEC), (104, nat.VX_MODE_VEC), (110, nat.VX_MODE_CC),
+                                    (91, nat.VX_MODE_VEC), (162, nat.VX_MODE_PROB)])
 def test_static_plans_identical(gpu, n, mode):
-    """Grid edges with compile-time stage plans (175 = 5 5 7, 128 = 8 8 2):
-    the slab and x-pass kernels built on them give bit-identical scores and
-    translations to the runtime-descriptor kernels, in the one-group modes
-    and in prob mode (two groups, mixed score); at 128 the rows also match
-    the FP64 oracle."""
+    """Grid edges with compile-time stage plans (175 = 5 5 7, 128 = 8 8 2,
+    96 = 8 4 3, 104 = 8 13, 110 = 2 5 11, 91 = 7 13 with a runtime x pass,
+    162 = 2 3 3 3 3): the slab and x-pass kernels built on them give the
+    runtime-descriptor kernels' scores and translations, in the one-group
+    modes and in prob mode (two groups, mixed score).  The FFT code may
+    contract a*b + c differently in the two forms, so scores agree to fp32
+    rounding (1e-5 relative here); at 175 they are bit-identical.  At 128 the
+    rows also match the FP64 oracle."""
     rng = np.random.default_rng(5700 + n + mode)
     c = n / 2
     g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
@@ -489,7 +494,11 @@ def test_static_plans_identical(gpu, n, mode):
             plan.set_prob_stats(0.4, 2.0, 0.1, 3.0, 0.2, enable=True)
         b, t = plan.score_rows(rot)
         res.append((b.cpu().numpy(), t.cpu().numpy()))
-    assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    if n == 175:
+        assert np.array_equal(res[0][0], res[1][0]) and np.array_equal(res[0][1], res[1][1])
+    else:
+        assert np.allclose(res[0][0], res[1][0], rtol=1e-5, atol=0)
+        assert np.array_equal(res[0][1], res[1][1])
     if n == 128 and mode == nat.VX_MODE_VEC:
         o32, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref32")
         o64, _ = _oracle_rows(td, tv, sd, sv, angles[:2], "VecProduct", "ref64")

@@ -390,6 +390,7 @@ struct StaticTile {
     static constexpr bool kStatic = true;
     static constexpr int kN = (Rs * ...);
     static constexpr int kNL = NL;
+    static constexpr unsigned kMask = (radix_bit(Rs) | ...);
     // Forward FFT of the NL lines of a column tile held in a (ping-pong with
     // b) by an NT-thread block; ends synchronised and returns the buffer
     // holding the result.
@@ -518,6 +519,7 @@ struct DynamicTile {
     static constexpr bool kStatic = false;
     static constexpr int kN = 0;
     static constexpr int kNL = 0;
+    static constexpr unsigned kMask = 0;
 };
 
 template <class SP, unsigned RM, int NT>

@@ -57,11 +57,20 @@ static bool plan_matches(int n) {
 
 // Grid edges with compile-time stage plans (radices in fft_desc_create's
 // order): kernels instantiated for these lengths carry every stage's radix,
-// span and twiddle offset as constants.  175 = 5 * 5 * 7 is the grid edge of
-// a 200^3 map at 2 A with the reference's padding; 128 = 8 * 8 * 2.
+// span and twiddle offset as constants.  The list is every FFTW-fast length
+// (2^a 3^b 5^c 7^d 11^e 13^f, e + f <= 1) from 90 to 180, the lengths whose
+// (y, z) slab takes the 16 x 1024 workgroup shape; 175 = 5 * 5 * 7 is the
+// grid edge of a 200^3 map at 2 A with the reference's padding.
 template <class... Ps>
 struct PlanList {};
-using StaticLengths = PlanList<StaticPlan<5, 5, 7>, StaticPlan<8, 8, 2>>;
+using StaticLengths = PlanList<StaticPlan<2, 3, 3, 5>, StaticPlan<7, 13>, StaticPlan<8, 4, 3>,
+    StaticPlan<2, 7, 7>, StaticPlan<3, 3, 11>, StaticPlan<4, 5, 5>, StaticPlan<8, 13>, StaticPlan<3, 5, 7>,
+    StaticPlan<4, 3, 3, 3>, StaticPlan<2, 5, 11>, StaticPlan<8, 2, 7>, StaticPlan<3, 3, 13>,
+    StaticPlan<8, 3, 5>, StaticPlan<5, 5, 5>, StaticPlan<2, 3, 3, 7>, StaticPlan<8, 8, 2>,
+    StaticPlan<2, 5, 13>, StaticPlan<4, 3, 11>, StaticPlan<3, 3, 3, 5>, StaticPlan<4, 5, 7>,
+    StaticPlan<8, 2, 3, 3>, StaticPlan<3, 7, 7>, StaticPlan<2, 3, 5, 5>, StaticPlan<2, 7, 11>,
+    StaticPlan<4, 3, 13>, StaticPlan<8, 4, 5>, StaticPlan<2, 3, 3, 3, 3>, StaticPlan<3, 5, 11>,
+    StaticPlan<8, 3, 7>, StaticPlan<5, 5, 7>, StaticPlan<8, 2, 11>, StaticPlan<4, 3, 3, 5>>;
 
 // Calls f(SP{}) for the listed plan whose length is n; false if none is.
 template <class F, class... Ps>
@@ -770,8 +779,18 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 // channel order, then its inverse), so one accumulator set serves every
 // group: the same arithmetic in the same order as accumulating all groups
 // at once.
+// 5 waves per SIMD (96 VGPRs) hold without spills when the tile values per
+// thread are at most 6 and the radices are small: radix 11 or 13 stages, or
+// radix 7 and 8 together, need more registers and keep the compiler's choice.
+constexpr int xcorr_waves_hint(int PF, unsigned mask) {
+    return (PF <= 6 && (mask & (radix_bit(11) | radix_bit(13))) == 0 &&
+            (mask & (radix_bit(7) | radix_bit(8))) != (radix_bit(7) | radix_bit(8)))
+               ? 5
+               : 1;
+}
+
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(PF <= 6 ? 5 : 1)))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(xcorr_waves_hint(PF, SP::kStatic ? SP::kMask : RM))))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w) {
     // a compile-time tile plan fixes the grid edge and the tile width
