@@ -2,7 +2,9 @@
 24,624 rows at -A 10).  The whole exhaustive search runs on the GPU; the
 oracle re-scores the GPU's top-10 rows and the next ten, so the top-10
 (rotation, translation) and their scores are checked against the
-reference's float32 arithmetic (ref32) and the best row against FP64."""
+reference's float32 arithmetic (ref32) and the best row against FP64.  The
+same in prob mode (four probability channels per map, the alpha-mixed
+z-score ranking the rows)."""
 import contextlib
 import io
 import os
@@ -69,3 +71,71 @@ def test_pair200_top10_against_oracle(gpu):
     print("[fp64] best row", r0, "gpu", float(b[r0]), "fp64", float(b64), "rel",
           abs(float(b[r0]) - float(b64)) / abs(float(b64)), flush=True)
     assert abs(b[r0] - b64) <= RTOL * abs(b64) and tuple(int(v) for v in t[r0]) == tuple(t64)
+
+
+def _pair200():
+    pair = synth.make_config_pair("pair200", 2001)
+    tmp = tempfile.mkdtemp(prefix="vx_full_")
+    paths = synth.write_pair(pair, os.path.join(tmp, "p"))
+    with contextlib.redirect_stdout(io.StringIO()):
+        m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], pair.a.threshold, pair.b.threshold, 2.0, 16.0)
+        n1 = V.fastVEC(m1, n1, 16.0)
+        n2 = V.fastVEC(m2, n2, 16.0)
+    return n1, n2
+
+
+def _probs(m, seed):
+    """Four probability channels on the map's support: per-voxel fractions
+    that sum to one where the density is non-zero (synthetic)."""
+    rng = np.random.default_rng(seed)
+    d = np.asarray(m.data)
+    u = rng.uniform(size=(4,) + d.shape).astype(np.float32) + np.float32(1e-3)
+    u /= u.sum(axis=0, keepdims=True)
+    return (u * (d != 0)).astype(np.float32)
+
+
+def test_pair200_prob_mode_top10_against_oracle(gpu):
+    n1, n2 = _pair200()
+    n = int(n1.xdim)
+    assert n == 175
+    tp, sp = _probs(n1, 11), _probs(n2, 12)
+    angles = calc_angle_comb(10.0)
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    plan = engine.SearchPlan(n, nat.VX_MODE_PROB, batch=16)
+    plan.set_target(*n1._dev, engine.to_dev(tp, torch.float32), 0.0)
+    plan.set_search(*n2._dev, engine.to_dev(sp, torch.float32), 0.0)
+    print("[case] pair200 prob pass 1", flush=True)
+    plan.set_prob_stats(0.5, 1.0, 0.0, 1.0, 0.0, enable=False)
+    b1, _ = plan.score_rows(rot)
+    b1 = b1.double().cpu().numpy()
+    vave, vstd = float(b1[:, 0].mean()), float(b1[:, 0].std())
+    pave, pstd = float(b1[:, 1].mean()), float(b1[:, 1].std())
+    alpha = 0.5
+    print("[case] pair200 prob pass 2 (mixed)", flush=True)
+    plan.set_prob_stats(alpha, vstd, vave, pstd, pave, enable=True)
+    best, trans = plan.score_rows(rot)
+    rows = engine.topn(best[:, 2].contiguous(), 20)
+    b = best.cpu().numpy()
+    t = trans.cpu().numpy()
+    tv = np.asarray(n1.vec, np.float32)
+    spec = ref.target_spectra([tv[..., 0], tv[..., 1], tv[..., 2]] + [tp[k] for k in range(4)])
+    sdata = np.asarray(n2.data, np.float64)
+    oracle = {}
+    for r in rows:
+        print("[case] prob oracle row", int(r), flush=True)
+        nv, nd, q1, q2, q3, q4 = ref.rot_mrc_prob(sdata, n2.vec, *[sp[k] for k in range(4)], angles[r],
+                                                  prec="ref32")
+        rv = ref.fft_correlate(spec[:3], [nv[..., 0], nv[..., 1], nv[..., 2]], "ref32")
+        rp = ref.fft_correlate(spec[3:], [q1, q2, q3, q4], "ref32")
+        ms, mt = ref.find_best_trans_mixed(rv, rp, alpha, vstd, vave, pstd, pave, prec="ref32")
+        vs, vt = ref.find_best_trans_list(rv)
+        oracle[int(r)] = (float(ms), tuple(mt), float(vs), tuple(vt))
+    for r in rows[:10]:
+        ms, mt, vs, vt = oracle[int(r)]
+        assert abs(b[r, 2] - ms) <= RTOL * max(abs(ms), 1.0), (r, b[r, 2], ms)
+        assert tuple(int(v) for v in t[r, 2]) == mt, (r, t[r, 2], mt)
+        assert abs(b[r, 0] - vs) <= RTOL * abs(vs), (r, b[r, 0], vs)
+        assert tuple(int(v) for v in t[r, 0]) == vt, (r, t[r, 0], vt)
+    s10 = min(oracle[int(r)][0] for r in rows[:10])
+    for r in rows[10:]:
+        assert oracle[int(r)][0] <= s10 + RTOL * max(abs(s10), 1.0), (r, oracle[int(r)][0], s10)
