@@ -87,10 +87,12 @@ def prepare(workload: str):
     from vesper_python_amd import synth
     from vesper_python_amd import VESPER_1_prob_new as V
     from vesper_python_amd import main as cli
-    w = WORKLOADS[workload]
+    w = dict(WORKLOADS[workload])
     pair = synth.make_config_pair(w["pair"], w["seed"])
     tmp = tempfile.mkdtemp(prefix="vx_bench_")
     paths = synth.write_pair(pair, os.path.join(tmp, "p"))
+    w["paths"] = paths
+    w["thresholds"] = (float(pair.a.threshold), float(pair.b.threshold))
     with contextlib.redirect_stdout(io.StringIO()):
         m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], pair.a.threshold, pair.b.threshold,
                                             w["s"], w["g"])
@@ -313,6 +315,24 @@ def measure_uneven_pairs(npairs: int = 6, ang: float = 30.0, s: float = 1.5, g: 
                     "the timed call"}
 
 
+def measure_cli(w):
+    """The drop-in CLI end to end on the workload's MRC files, as a user runs
+    it: `main orig -a A.mrc -b B.mrc -t -T -g -s -A -N 10` (MRC reading,
+    resampling, fastVEC, the exhaustive search, local refinement, FP64
+    get_score of the models and the reference's printed output)."""
+    from vesper_python_amd import main as cli
+    t1, t2 = w["thresholds"]
+    argv = ["orig", "-a", w["paths"]["A"], "-b", w["paths"]["B"], "-t", str(t1), "-T", str(t2),
+            "-g", str(w["g"]), "-s", str(w["s"]), "-A", str(w["A"]), "-N", "10"]
+    buf = io.StringIO()
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(buf):
+        res = cli.main(argv)
+    dt = time.perf_counter() - t0
+    return {"seconds": round(dt, 3), "models": len(res), "printed_lines": buf.getvalue().count("\n"),
+            "argv": " ".join(["orig", "-a", "A.mrc", "-b", "B.mrc"] + argv[5:])}
+
+
 def measure_interp(edge: int = 337, cpu: bool = True):
     """Prediction-grid interpolation (interpolation_functions.py:6-135) on a
     synthetic [edge]^3 x 4 float32 grid with values on one parity lattice,
@@ -406,6 +426,7 @@ def main():
     ap.add_argument("--no-interp", action="store_true", help="skip the prediction-grid interpolation measurement")
     ap.add_argument("--no-libfft", action="store_true", help="skip the vendor-FFT pipeline reference point")
     ap.add_argument("--no-pairs", action="store_true", help="skip the uneven multi-pair measurement")
+    ap.add_argument("--no-cli", action="store_true", help="skip the end-to-end CLI measurement")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -520,6 +541,10 @@ def main():
     if not args.no_libfft and not args.rows:
         libfft = measure_library_fft(n1, n2, rot, args.batch, plan)
 
+    cli_m = None
+    if not args.no_cli and not args.rows:
+        cli_m = measure_cli(w)
+
     pairs_m = None
     if not args.no_pairs and not args.rows:
         pairs_m = measure_uneven_pairs()
@@ -554,7 +579,8 @@ def main():
                       "batch": args.batch, "seed": w["seed"], "top_rows": [int(v) for v in top[:10]],
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
-           "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m}
+           "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m,
+           "cli_end_to_end": cli_m}
     if cpu:
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
     if cpu_all:
