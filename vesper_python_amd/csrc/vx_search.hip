@@ -1673,7 +1673,9 @@ static void launch_reg_pf_nt(dim3 grid, size_t lds, hipStream_t st, const float2
 
 // Smallest workgroup (64, 128 or 256 threads) holding a tile of n * T values
 // at kXcorrPF per thread: tiles of few columns run as one- or two-wave
-// workgroups whose barriers stall only their own waves.
+// workgroups whose barriers stall only their own waves.  256-thread tiles of
+// at most 1536 values take 6 values per thread (fewer registers: 5 waves per
+// SIMD instead of 4).
 template <int G, unsigned RM>
 static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                           float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
