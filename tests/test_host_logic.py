@@ -114,12 +114,14 @@ def _fftw_fast(n):
     return m == 1 and e <= 1
 
 
-def test_static_lengths_follow_the_descriptor_order():
+@pytest.mark.parametrize("name,lo,hi", [("StaticLengths", 90, 180), ("SmallLengths", 16, 89)])
+def test_static_lengths_follow_the_descriptor_order(name, lo, hi):
     """Every compile-time plan lists its radices in fft_desc_create's order
-    (else the host check would silently fall back to the runtime plan), and
-    the list is every FFTW-fast length from 90 to 180."""
+    (else the host check would silently fall back to the runtime plan); the
+    16 x 1024-slab list is every FFTW-fast length from 90 to 180, the
+    8 x 512 one every such length from 16 to 89 with two stages or more."""
     src = open(os.path.join(ROOT, "vesper_python_amd", "csrc", "vx_search.hip")).read()
-    m = re.search(r"using StaticLengths = PlanList<(.*?)>;", src, re.S)
+    m = re.search(r"using %s = PlanList<(.*?)>;" % name, src, re.S)
     assert m
     plans = [[int(v) for v in g.split(",")] for g in re.findall(r"StaticPlan<([0-9, ]+)>", m.group(1))]
     lengths = []
@@ -127,4 +129,5 @@ def test_static_lengths_follow_the_descriptor_order():
         n = int(np.prod(radices))
         assert radices == _factorize(n), (n, radices)
         lengths.append(n)
-    assert lengths == [n for n in range(90, 181) if _fftw_fast(n)]
+    want = [n for n in range(lo, hi + 1) if _fftw_fast(n) and (hi > 89 or len(_factorize(n)) >= 2)]
+    assert lengths == want

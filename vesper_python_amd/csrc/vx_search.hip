@@ -57,7 +57,7 @@ static bool plan_matches(int n) {
 
 // Grid edges with compile-time stage plans (radices in fft_desc_create's
 // order): kernels instantiated for these lengths carry every stage's radix,
-// span and twiddle offset as constants.  The list is every FFTW-fast length
+// span and twiddle offset as constants.  This list is every FFTW-fast length
 // (2^a 3^b 5^c 7^d 11^e 13^f, e + f <= 1) from 90 to 180, the lengths whose
 // (y, z) slab takes the 16 x 1024 workgroup shape; 175 = 5 * 5 * 7 is the
 // grid edge of a 200^3 map at 2 A with the reference's padding.
@@ -71,6 +71,18 @@ using StaticLengths = PlanList<StaticPlan<2, 3, 3, 5>, StaticPlan<7, 13>, Static
     StaticPlan<8, 2, 3, 3>, StaticPlan<3, 7, 7>, StaticPlan<2, 3, 5, 5>, StaticPlan<2, 7, 11>,
     StaticPlan<4, 3, 13>, StaticPlan<8, 4, 5>, StaticPlan<2, 3, 3, 3, 3>, StaticPlan<3, 5, 11>,
     StaticPlan<8, 3, 7>, StaticPlan<5, 5, 7>, StaticPlan<8, 2, 11>, StaticPlan<4, 3, 3, 5>>;
+
+// The FFTW-fast lengths from 16 to 89 with two stages or more: their slab
+// takes the 8 x 512 workgroup shape.
+using SmallLengths = PlanList<StaticPlan<8, 2>, StaticPlan<2, 3, 3>, StaticPlan<4, 5>, StaticPlan<3, 7>,
+    StaticPlan<2, 11>, StaticPlan<8, 3>, StaticPlan<5, 5>, StaticPlan<2, 13>, StaticPlan<3, 3, 3>,
+    StaticPlan<4, 7>, StaticPlan<2, 3, 5>, StaticPlan<8, 4>, StaticPlan<3, 11>, StaticPlan<5, 7>,
+    StaticPlan<4, 3, 3>, StaticPlan<3, 13>, StaticPlan<8, 5>, StaticPlan<2, 3, 7>, StaticPlan<4, 11>,
+    StaticPlan<3, 3, 5>, StaticPlan<8, 2, 3>, StaticPlan<7, 7>, StaticPlan<2, 5, 5>, StaticPlan<4, 13>,
+    StaticPlan<2, 3, 3, 3>, StaticPlan<5, 11>, StaticPlan<8, 7>, StaticPlan<4, 3, 5>, StaticPlan<3, 3, 7>,
+    StaticPlan<8, 8>, StaticPlan<5, 13>, StaticPlan<2, 3, 11>, StaticPlan<2, 5, 7>, StaticPlan<8, 3, 3>,
+    StaticPlan<3, 5, 5>, StaticPlan<7, 11>, StaticPlan<2, 3, 13>, StaticPlan<8, 2, 5>, StaticPlan<3, 3, 3, 3>,
+    StaticPlan<4, 3, 7>, StaticPlan<8, 11>>;
 
 // Calls f(SP{}) for the listed plan whose length is n; false if none is.
 template <class F, class... Ps>
@@ -1585,7 +1597,11 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
-    if (maxv == 8)       VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
+    if (maxv == 8 && use_static && with_static_plan(n, SmallLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_SLAB_LAUNCH_SP(k_fwd_zy, 8, 512, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+        })) {
+    } else if (maxv == 8) VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
@@ -1609,7 +1625,11 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
-    if (maxv == 8)       VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    if (maxv == 8 && use_static && with_static_plan(n, SmallLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_SLAB_LAUNCH_SP(k_inv_yz, 8, 512, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+        })) {
+    } else if (maxv == 8) VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -1683,17 +1703,17 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
     // compile-time tile plans: 8 columns per tile, every tile full (the
     // grid edges listed have n (n / 2 + 1) divisible by 8), with the same
     // threads and values per thread as the runtime form below
+    auto launch_static = [&](auto sp) {
+        using SP = decltype(sp);
+        constexpr int E = SP::kN * 8;
+        constexpr int NT = E <= 64 * kXcorrPF ? 64 : (E <= 128 * kXcorrPF ? 128 : 256);
+        constexpr int PF = (NT == 256 && E <= 256 * 6) ? 6 : kXcorrPF;
+        static_assert(E <= NT * PF, "tile exceeds the workgroup's values");
+        launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(grid, lds, st, spec, tgt, out, fd, n,
+                                                                             nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    };
     if (use_static && T == 8 && (n * nhz) % 8 == 0 &&
-        with_static_plan(n, StaticLengths{}, [&](auto sp) {
-            using SP = decltype(sp);
-            constexpr int E = SP::kN * 8;
-            constexpr int NT = E <= 64 * kXcorrPF ? 64 : (E <= 128 * kXcorrPF ? 128 : 256);
-            constexpr int PF = (NT == 256 && E <= 256 * 6) ? 6 : kXcorrPF;
-            static_assert(E <= NT * PF, "tile exceeds the workgroup's values");
-            launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(grid, lds, st, spec, tgt, out, fd,
-                                                                                 n, nhz, nb, C, T, ntiles, grp_mask,
-                                                                                 lo, w);
-        }))
+        (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static)))
         return;
     if (n * T <= 64 * kXcorrPF)
         launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
