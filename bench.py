@@ -333,6 +333,34 @@ def measure_cli(w):
             "argv": " ".join(["orig", "-a", "A.mrc", "-b", "B.mrc"] + argv[5:])}
 
 
+def measure_cli_prob(w):
+    """The drop-in CLI's prob mode end to end on the same pair with four
+    synthetic probability channels per map (Dirichlet(0.5) fractions on the
+    density mask, smoothed, renormalised; written as the .npy files the CLI
+    reads): `main prob -a A.mrc -npa A.npy -b B.mrc -npb B.npy -t -T -g -s
+    -A -N 10 -alpha 0.5` -- ten resampled and fastVEC'd maps, the statistics
+    pass and the mixing pass over every row, refinement, scoring, output."""
+    import numpy as np
+    from vesper_python_amd import main as cli
+    from vesper_python_amd import synth
+    pair = synth.make_config_pair(w["pair"], w["seed"])
+    rng = np.random.default_rng(w["seed"] + 1)
+    synth.add_prob_channels(rng, pair.a)
+    synth.add_prob_channels(rng, pair.b)
+    tmp = tempfile.mkdtemp(prefix="vx_prob_")
+    paths = synth.write_pair(pair, os.path.join(tmp, "p"))
+    argv = ["prob", "-a", paths["A"], "-npa", paths["A_prob"], "-b", paths["B"], "-npb", paths["B_prob"],
+            "-t", str(pair.a.threshold), "-T", str(pair.b.threshold), "-g", str(w["g"]), "-s", str(w["s"]),
+            "-A", str(w["A"]), "-N", "10", "-alpha", "0.5"]
+    buf = io.StringIO()
+    t0 = time.perf_counter()
+    with contextlib.redirect_stdout(buf):
+        res = cli.main(argv)
+    dt = time.perf_counter() - t0
+    return {"seconds": round(dt, 3), "models": len(res), "printed_lines": buf.getvalue().count("\n"),
+            "argv": "prob -a A.mrc -npa A.npy -b B.mrc -npb B.npy " + " ".join(argv[9:])}
+
+
 def measure_interp(edge: int = 337, cpu: bool = True):
     """Prediction-grid interpolation (interpolation_functions.py:6-135) on a
     synthetic [edge]^3 x 4 float32 grid with values on one parity lattice,
@@ -551,6 +579,7 @@ def main():
     cli_m = None
     if not args.no_cli and not args.rows:
         cli_m = measure_cli(w)
+        cli_m["prob"] = measure_cli_prob(w)
 
     pairs_m = None
     if not args.no_pairs and not args.rows:
