@@ -113,7 +113,7 @@ int vx_plan_destroy(vx_plan* plan);
                              second stream is forked from and joined to the caller's);
                              1 = one (default).  Identical results. */
 #define VX_TUNE_STATIC_PLANS 8 /* 1 (default): grid edges with compile-time FFT stage plans
-                                  (every FFTW-fast length from 16 to 180) run kernels with
+                                  (every FFTW-fast length from 16 to 200) run kernels with
                                   every stage's radix, span and twiddle offset as constants;
                                   0 = runtime descriptors only.  Results equal to fp32
                                   rounding (bit-identical at 175). */

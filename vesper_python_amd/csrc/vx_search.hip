@@ -84,6 +84,11 @@ using SmallLengths = PlanList<StaticPlan<8, 2>, StaticPlan<2, 3, 3>, StaticPlan<
     StaticPlan<3, 5, 5>, StaticPlan<7, 11>, StaticPlan<2, 3, 13>, StaticPlan<8, 2, 5>, StaticPlan<3, 3, 3, 3>,
     StaticPlan<4, 3, 7>, StaticPlan<8, 11>>;
 
+// The FFTW-fast lengths from 181 to 200: their slab takes the 24 x 1024
+// workgroup shape (above 200 the slab exceeds LDS and the pencils run).
+using MediumLengths = PlanList<StaticPlan<2, 7, 13>, StaticPlan<3, 3, 3, 7>, StaticPlan<8, 8, 3>,
+                               StaticPlan<3, 5, 13>, StaticPlan<4, 7, 7>, StaticPlan<2, 3, 3, 11>, StaticPlan<8, 5, 5>>;
+
 // Calls f(SP{}) for the listed plan whose length is n; false if none is.
 template <class F, class... Ps>
 static bool with_static_plan(int n, PlanList<Ps...>, F&& f) {
@@ -1611,7 +1616,11 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
-    else                 VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz, lo, w);
+    else if (maxv == 24 && use_static && with_static_plan(n, MediumLengths{}, [&](auto sp) {
+                 using SP = decltype(sp);
+                 VX_SLAB_LAUNCH_SP(k_fwd_zy, 24, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+             })) {
+    } else               VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz, lo, w);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -1639,7 +1648,11 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
-    else                 VX_SLAB_LAUNCH(k_inv_yz, 24, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
+    else if (maxv == 24 && use_static && with_static_plan(n, MediumLengths{}, [&](auto sp) {
+                 using SP = decltype(sp);
+                 VX_SLAB_LAUNCH_SP(k_inv_yz, 24, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+             })) {
+    } else               VX_SLAB_LAUNCH(k_inv_yz, 24, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -1713,7 +1726,8 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
                                                                              nhz, nb, C, T, ntiles, grp_mask, lo, w);
     };
     if (use_static && T == 8 && (n * nhz) % 8 == 0 &&
-        (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static)))
+        (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static) ||
+         with_static_plan(n, MediumLengths{}, launch_static)))
         return;
     if (n * T <= 64 * kXcorrPF)
         launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);

@@ -182,7 +182,7 @@ class SearchPlan:
              static_plans: bool | None = None):
         """Select kernel schedules.  x-pass variants 0, 1 and 3 (FFT) and the
         slab shape give identical results; the compile-time stage plans (grid
-        edges 16..180) agree to fp32 rounding (bit-identical at 175); the
+        edges 16..200) agree to fp32 rounding (bit-identical at 175); the
         pencil passes and variant 2 (split-bf16 MFMA DFT) agree within 1e-4
         relative."""
         if static_plans is not None:
