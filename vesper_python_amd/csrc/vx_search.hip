@@ -809,7 +809,8 @@ constexpr int xcorr_waves_hint(int PF, unsigned mask) {
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
 __global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(xcorr_waves_hint(PF, SP::kStatic ? SP::kMask : RM))))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
-               FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w) {
+               FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w,
+               int tile0) {
     // a compile-time tile plan fixes the grid edge and the tile width
     const int n = SP::kStatic ? SP::kN : n_;
     const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
@@ -819,6 +820,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     extern __shared__ float2 lds[];
     int b, tile;
     if (!xcd_tile(blockIdx.x, nb, ntiles, &tile, &b)) return;
+    tile += tile0;   // tiles [tile0, tile0 + ntiles) of the plane
     float2* A = lds;
     float2* Bf = lds + n * T;
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
@@ -1696,47 +1698,69 @@ static void launch_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec,
 template <int G, unsigned RM, int NT, int PF = kXcorrPF, class SP = DynamicTile>
 static void launch_reg_pf_nt(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                              float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
-                             unsigned grp_mask, int lo, int w) {
+                             unsigned grp_mask, int lo, int w, int tile0 = 0) {
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void*)k_xcorr_reg_pf<G, PF, RM, NT, SP>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL((k_xcorr_reg_pf<G, PF, RM, NT, SP>), grid, dim3(NT), lds, st, spec, tgt, out, fd, n, nhz,
-                       nb, C, T, ntiles, grp_mask, lo, w);
+                       nb, C, T, ntiles, grp_mask, lo, w, tile0);
 }
 
-// Smallest workgroup (64, 128 or 256 threads) holding a tile of n * T values
-// at kXcorrPF per thread: tiles of few columns run as one- or two-wave
-// workgroups whose barriers stall only their own waves.  256-thread tiles of
+// grid of the x pass over ntiles tiles of nb rows (whole XCD groups of 8)
+static dim3 xcorr_grid(int ntiles, int nb) { return dim3((unsigned)(((ntiles + 7) / 8) * 8 * nb)); }
+
+// Runtime-plan form over tiles [tile0, tile0 + ntiles): the smallest
+// workgroup (64, 128 or 256 threads) holding a tile of n * T values at
+// kXcorrPF per thread, so tiles of few columns run as one- or two-wave
+// workgroups whose barriers stall only their own waves; 256-thread tiles of
 // at most 1536 values take 6 values per thread (fewer registers: 5 waves per
 // SIMD instead of 4).
+template <int G, unsigned RM>
+static void launch_reg_pf_dyn(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
+                              float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
+                              unsigned grp_mask, int lo, int w, int tile0) {
+    if (n * T <= 64 * kXcorrPF)
+        launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w,
+                                    tile0);
+    else if (n * T <= 128 * kXcorrPF)
+        launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w,
+                                     tile0);
+    else if (n * T <= 256 * 6)
+        launch_reg_pf_nt<G, RM, 256, 6>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w,
+                                        tile0);
+    else
+        launch_reg_pf_nt<G, RM, 256>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w,
+                                     tile0);
+}
+
+// The register x pass over all ntiles tiles of the plane.
 template <int G, unsigned RM>
 static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                           float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
                           unsigned grp_mask, int lo, int w, bool use_static) {
-    // compile-time tile plans: 8 columns per tile, every tile full (the
-    // grid edges listed have n (n / 2 + 1) divisible by 8), with the same
-    // threads and values per thread as the runtime form below
+    // compile-time tile plans: the full 8-column tiles, with the same threads
+    // and values per thread as the runtime form; a last partial tile (when
+    // n (n / 2 + 1) is not a multiple of 8) runs the runtime form
+    const int full = n * nhz / 8;
     auto launch_static = [&](auto sp) {
         using SP = decltype(sp);
         constexpr int E = SP::kN * 8;
         constexpr int NT = E <= 64 * kXcorrPF ? 64 : (E <= 128 * kXcorrPF ? 128 : 256);
         constexpr int PF = (NT == 256 && E <= 256 * 6) ? 6 : kXcorrPF;
         static_assert(E <= NT * PF, "tile exceeds the workgroup's values");
-        launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(grid, lds, st, spec, tgt, out, fd, n,
-                                                                             nhz, nb, C, T, ntiles, grp_mask, lo, w);
+        launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(xcorr_grid(full, nb), lds, st, spec,
+                                                                             tgt, out, fd, n, nhz, nb, C, T, full,
+                                                                             grp_mask, lo, w, 0);
     };
-    if (use_static && T == 8 && (n * nhz) % 8 == 0 &&
+    if (use_static && T == 8 && full > 0 &&
         (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static) ||
-         with_static_plan(n, MediumLengths{}, launch_static)))
+         with_static_plan(n, MediumLengths{}, launch_static))) {
+        if (full < ntiles)
+            launch_reg_pf_dyn<G, RM>(xcorr_grid(ntiles - full, nb), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T,
+                                     ntiles - full, grp_mask, lo, w, full);
         return;
-    if (n * T <= 64 * kXcorrPF)
-        launch_reg_pf_nt<G, RM, 64>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-    else if (n * T <= 128 * kXcorrPF)
-        launch_reg_pf_nt<G, RM, 128>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-    else if (n * T <= 256 * 6)
-        launch_reg_pf_nt<G, RM, 256, 6>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
-    else
-        launch_reg_pf_nt<G, RM, 256>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w);
+    }
+    launch_reg_pf_dyn<G, RM>(grid, lds, st, spec, tgt, out, fd, n, nhz, nb, C, T, ntiles, grp_mask, lo, w, 0);
 }
 
 int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDesc& fd, int n,
