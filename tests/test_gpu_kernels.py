@@ -461,12 +461,13 @@ def test_slab_shapes_identical_at_n175(gpu):
                                     (91, nat.VX_MODE_VEC), (162, nat.VX_MODE_PROB),
                                     (24, nat.VX_MODE_CC), (45, nat.VX_MODE_VEC), (64, nat.VX_MODE_PROB),
                                     (80, nat.VX_MODE_VEC), (77, nat.VX_MODE_VEC),
-                                    (200, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC)])
+                                    (200, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (250, nat.VX_MODE_VEC)])
 def test_static_plans_identical(gpu, n, mode):
     """Grid edges with compile-time stage plans (175 = 5 5 7, 128 = 8 8 2,
     96 = 8 4 3, 104 = 8 13, 110 = 2 5 11, 91 = 7 13 with a runtime x pass,
     162 = 2 3 3 3 3; in the 8 x 512 slab shape 24, 45, 64, 80, 77 = 7 11;
-    in the 24 x 1024 shape 200 = 8 5 5, 182 = 2 7 13):
+    in the 24 x 1024 shape 200 = 8 5 5, 182 = 2 7 13; 250 = 2 5 5 5 on the
+    pencils with a compile-time x pass):
     the slab and x-pass kernels built on them give the
     runtime-descriptor kernels' scores and translations, in the one-group
     modes and in prob mode (two groups, mixed score).  The FFT code may

@@ -89,6 +89,14 @@ using SmallLengths = PlanList<StaticPlan<8, 2>, StaticPlan<2, 3, 3>, StaticPlan<
 using MediumLengths = PlanList<StaticPlan<2, 7, 13>, StaticPlan<3, 3, 3, 7>, StaticPlan<8, 8, 3>,
                                StaticPlan<3, 5, 13>, StaticPlan<4, 7, 7>, StaticPlan<2, 3, 3, 11>, StaticPlan<8, 5, 5>>;
 
+// The FFTW-fast lengths from 201 to 256: the pencil passes carry their
+// (y, z) transforms, and the register x pass (N x 8 <= 2048) its compile-time
+// tile plan.
+using TileLengths = PlanList<StaticPlan<8, 2, 13>, StaticPlan<2, 3, 5, 7>, StaticPlan<8, 3, 3, 3>,
+    StaticPlan<4, 5, 11>, StaticPlan<8, 4, 7>, StaticPlan<3, 3, 5, 5>, StaticPlan<3, 7, 11>,
+    StaticPlan<2, 3, 3, 13>, StaticPlan<8, 2, 3, 5>, StaticPlan<3, 3, 3, 3, 3>, StaticPlan<5, 7, 7>,
+    StaticPlan<2, 5, 5, 5>, StaticPlan<4, 3, 3, 7>, StaticPlan<8, 8, 4>>;
+
 // Calls f(SP{}) for the listed plan whose length is n; false if none is.
 template <class F, class... Ps>
 static bool with_static_plan(int n, PlanList<Ps...>, F&& f) {
@@ -1754,7 +1762,7 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
     };
     if (use_static && T == 8 && full > 0 &&
         (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static) ||
-         with_static_plan(n, MediumLengths{}, launch_static))) {
+         with_static_plan(n, MediumLengths{}, launch_static) || with_static_plan(n, TileLengths{}, launch_static))) {
         if (full < ntiles)
             launch_reg_pf_dyn<G, RM>(xcorr_grid(ntiles - full, nb), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T,
                                      ntiles - full, grp_mask, lo, w, full);
