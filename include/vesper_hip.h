@@ -272,7 +272,8 @@ int vx_sco_arr(int n, const float* tvec_dev, const float* svec_dev, const int32_
 /* C = op(A) op(B): A [M x K] (transA: stored K x M), B [K x N] (transB:
  * stored N x K), row-major bf16.  K a multiple of 16.
  * epilogue 0: C fp32 = AB;  1: C bf16 = relu(AB + bias);  2: C bf16 =
- * AB where aux (bf16 [M x N], ld ldaux) > 0 else 0;  3: C fp32 += AB (atomic). */
+ * AB where aux (bf16 [M x N], ld ldaux) > 0 else 0;  3: C fp32 += AB (one
+ * workgroup per element of C, so the result does not depend on timing). */
 int vx_gemm_bf16(int M, int N, int K, const void* A_dev, int lda, int transA, const void* B_dev, int ldb,
                  int transB, void* C_dev, int ldc, int epilogue, const float* bias_dev, const void* aux_dev,
                  int ldaux, vx_stream stream);
@@ -282,12 +283,13 @@ int vx_mlp_head_fwd(int M, int K, const void* X_dev, int ldx, const float* w_dev
                     float* pred_dev, vx_stream stream);
 
 /* its backward with mean-squared loss: dw += .., db += .., dX = d w (X > 0),
- * loss += mean (pred - y)^2 */
+ * loss += mean (pred - y)^2; the sums over rows run in a fixed order, so
+ * repeated calls give identical results */
 int vx_mlp_head_bwd(int M, int K, const void* X_dev, int ldx, const float* w_dev, const float* pred_dev,
                     const float* y_dev, float* dw_dev, float* db_dev, void* dX_dev, float* loss_dev,
                     vx_stream stream);
 
-/* out[n] += sum_m G[m][n] (bf16 G) */
+/* out[n] += sum_m G[m][n] (bf16 G), summed in a fixed order */
 int vx_colsum_bf16(int M, int N, const void* G_dev, int ldg, float* out_dev, vx_stream stream);
 
 /* Adam step (bias-corrected, step >= 1) on fp32 p; refreshes the bf16 copy if non-NULL */

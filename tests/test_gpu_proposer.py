@@ -96,6 +96,39 @@ def test_training_fits_a_smooth_function(gpu):
     assert losses[-1] < 0.2 * losses[0]
 
 
+def test_training_is_repeatable(gpu):
+    """The backward sums over rows run in a fixed order: gradients, trained
+    weights and moment descriptors are bit-identical across repeated runs."""
+    rng = np.random.default_rng(6)
+    M = 1000
+    Xh = rng.uniform(-1, 1, (M, proposer.IN)).astype(np.float32)
+    y = (np.sin(3 * Xh[:, 0]) + Xh[:, 1]).astype(np.float32)
+    X, yd = _bf(Xh), torch.from_numpy(y).cuda()
+    Xp = torch.zeros((proposer.pad64(M), proposer.IN), dtype=torch.bfloat16, device="cuda")
+    Xp[:M] = X
+    print("[case] repeatable grads M", M, flush=True)
+    mlp = proposer.RotationMLP(seed=3)
+    g1, l1 = mlp.grads(Xp, yd, M)
+    g2, l2 = mlp.grads(Xp, yd, M)
+    assert l1.item() == l2.item()
+    for a, b in zip(g1, g2):
+        assert torch.equal(a, b)
+    print("[case] repeatable training", flush=True)
+    runs = []
+    for _ in range(2):
+        m = proposer.RotationMLP(seed=3)
+        losses = m.fit(X, yd, M, epochs=4, batch=256, lr=2e-3, seed=1)
+        runs.append((losses, [p.cpu().clone() for p in m.params]))
+    assert runs[0][0] == runs[1][0]
+    for a, b in zip(runs[0][1], runs[1][1]):
+        assert torch.equal(a, b)
+    n = 24
+    data = rng.uniform(0, 1, (n, n, n)) * (rng.uniform(size=(n, n, n)) < 0.3)
+    vec = rng.standard_normal((n, n, n, 3)).astype(np.float32) * (data > 0)[..., None]
+    dd, dv = engine.to_dev(data, torch.float64), engine.to_dev(vec, torch.float32)
+    assert torch.equal(proposer.descriptors(dd, dv), proposer.descriptors(dd, dv))
+
+
 def test_descriptors_match_oracle(gpu):
     rng = np.random.default_rng(5)
     n = 20

@@ -116,34 +116,80 @@ __global__ void k_head_fwd(int M, int K, const unsigned short* __restrict__ X, i
     if (lane == 0) pred[m] = s + b[0];
 }
 
-// Backward of the output layer with the mean-squared loss:
-// d = 2 (pred - y) / M;  dw += sum_m d X[m];  db += sum_m d;
-// dX[m][k] = d w[k] * (X[m][k] > 0)  (through the ReLU that produced X).
-__global__ void k_head_bwd(int M, int K, const unsigned short* __restrict__ X, int ldx, const float* __restrict__ w,
-                           const float* __restrict__ pred, const float* __restrict__ y, float* __restrict__ dw,
-                           float* __restrict__ db, unsigned short* __restrict__ dX, float* __restrict__ loss) {
+// Backward of the output layer with the mean-squared loss, d = 2 (pred - y) / M:
+// dX[m][k] = d w[k] * (X[m][k] > 0)  (through the ReLU that produced X),
+// one wave per row.
+__global__ void k_head_bwd_rows(int M, int K, const unsigned short* __restrict__ X, int ldx,
+                                const float* __restrict__ w, const float* __restrict__ pred,
+                                const float* __restrict__ y, unsigned short* __restrict__ dX) {
     const int m = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (m >= M) return;
-    const float e = pred[m] - y[m];
-    const float d = 2.f * e / (float)M;
+    const float d = 2.f * (pred[m] - y[m]) / (float)M;
     for (int k = lane; k < K; k += 64) {
         const float xv = bf2f(X[(size_t)m * ldx + k]);
-        atomicAdd(&dw[k], d * xv);
         dX[(size_t)m * ldx + k] = f2bf(xv > 0.f ? d * w[k] : 0.f);
-    }
-    if (lane == 0) {
-        atomicAdd(db, d);
-        atomicAdd(loss, e * e / (float)M);
     }
 }
 
-__global__ void k_colsum_bf16(int M, int N, const unsigned short* __restrict__ G, int ldg, float* __restrict__ out) {
-    const int col = blockIdx.x * blockDim.x + threadIdx.x;
-    if (col >= N) return;
+// Row sums in a fixed order, so that repeated runs give identical results:
+// a 1024-thread workgroup holds 64 columns x 16 row phases; phase p sums rows
+// p, p + 16, ... in order, then phase 0 adds the 16 partials in order.
+constexpr int kSumPhases = 16;
+
+// dw[k] += sum_m d_m X[m][k];  db += sum_m d_m;  loss += sum_m e_m^2 / M
+// (e = pred - y, d = 2 e / M); db and loss come from workgroup 0.
+__global__ void __launch_bounds__(1024)
+k_head_bwd_sums(int M, int K, const unsigned short* __restrict__ X, int ldx, const float* __restrict__ pred,
+                const float* __restrict__ y, float* __restrict__ dw, float* __restrict__ db,
+                float* __restrict__ loss) {
+    __shared__ float red[3][kSumPhases][64];
+    const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + c;
+    float s = 0.f, sd = 0.f, se = 0.f;
+#pragma unroll 4
+    for (int m = ph; m < M; m += kSumPhases) {
+        const float e = pred[m] - y[m];
+        const float d = 2.f * e / (float)M;
+        if (k < K) s += d * bf2f(X[(size_t)m * ldx + k]);
+        sd += d;
+        se += e * e / (float)M;
+    }
+    red[0][ph][c] = s;
+    red[1][ph][c] = sd;
+    red[2][ph][c] = se;
+    __syncthreads();
+    if (ph != 0) return;
+    float a = 0.f, b = 0.f, l = 0.f;
+    for (int p = 0; p < kSumPhases; ++p) {
+        a += red[0][p][c];
+        b += red[1][p][c];
+        l += red[2][p][c];
+    }
+    if (k < K) dw[k] += a;
+    if (blockIdx.x == 0 && c == 0) {
+        db[0] += b;
+        loss[0] += l;
+    }
+}
+
+// out[col] += sum_m G[m][col]  (bias gradients), fixed summation order.
+__global__ void __launch_bounds__(1024)
+k_colsum_bf16(int M, int N, const unsigned short* __restrict__ G, int ldg, float* __restrict__ out) {
+    __shared__ float red[kSumPhases][64];
+    const int c = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + c;
     float s = 0.f;
-    for (int m = blockIdx.y; m < M; m += gridDim.y) s += bf2f(G[(size_t)m * ldg + col]);
-    atomicAdd(&out[col], s);
+    if (col < N) {
+#pragma unroll 8
+        for (int m = ph; m < M; m += kSumPhases) s += bf2f(G[(size_t)m * ldg + col]);
+    }
+    red[ph][c] = s;
+    __syncthreads();
+    if (ph != 0 || col >= N) return;
+    float a = 0.f;
+    for (int p = 0; p < kSumPhases; ++p) a += red[p][c];
+    out[col] += a;
 }
 
 __global__ void k_adam(int n, float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
@@ -195,19 +241,33 @@ k_moment_sums(int n, const double* __restrict__ data, const float* __restrict__ 
         s[23] += v[0] * v[0]; s[24] += v[1] * v[1]; s[25] += v[2] * v[2];
         s[26] += v[0] * v[1]; s[27] += v[0] * v[2]; s[28] += v[1] * v[2];
     }
+    // per-workgroup partials in a fixed order (out[blockIdx.x * 32 + k]);
+    // k_moment_finish adds the workgroups' partials in order
+    __shared__ double wsum[4][29];
 #pragma unroll
     for (int k = 0; k < 29; ++k) {
         double t = s[k];
         for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-        if ((threadIdx.x & 63) == 0) atomicAdd(&out[k], t);
+        if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6][k] = t;
     }
+    __syncthreads();
+    if (threadIdx.x < 29)
+        out[(size_t)blockIdx.x * 32 + threadIdx.x] =
+            ((wsum[0][threadIdx.x] + wsum[1][threadIdx.x]) + wsum[2][threadIdx.x]) + wsum[3][threadIdx.x];
 }
 
 // sums -> 32 descriptors: centroid (3), second moments (6), mean vector (3),
 // position-vector moments (9), vector second moments (6), invariants (5:
 // trace of second moments, |centroid|, |mean vector|, trace of r v^T,
 // occupancy)
-__global__ void k_moment_finish(int n, const double* __restrict__ s, float* __restrict__ d) {
+__global__ void k_moment_finish(int n, int nblk, const double* __restrict__ part, float* __restrict__ d) {
+    __shared__ double s[29];
+    if (threadIdx.x < 29) {
+        double t = 0.0;
+        for (int b = 0; b < nblk; ++b) t += part[(size_t)b * 32 + threadIdx.x];
+        s[threadIdx.x] = t;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     const double w = s[0] > 0 ? s[0] : 1.0, c = s[10] > 0 ? s[10] : 1.0;
     for (int a = 0; a < 3; ++a) d[a] = (float)(s[1 + a] / w);
@@ -308,15 +368,20 @@ extern "C" int vx_mlp_head_fwd(int M, int K, const void* X, int ldx, const float
 extern "C" int vx_mlp_head_bwd(int M, int K, const void* X, int ldx, const float* w, const float* pred,
                                const float* y, float* dw, float* db, void* dX, float* loss, vx_stream stream) {
     VX_REQUIRE(X && w && pred && y && dw && db && dX && loss && M > 0, VX_ERR_ARG, "bad arguments");
-    hipLaunchKernelGGL(k_head_bwd, dim3((M + 3) / 4), dim3(256), 0, (hipStream_t)stream, M, K,
-                       (const unsigned short*)X, ldx, w, pred, y, dw, db, (unsigned short*)dX, loss);
+    VX_REQUIRE(K > 0, VX_ERR_ARG, "bad arguments");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_head_bwd_rows, dim3((M + 3) / 4), dim3(256), 0, st, M, K, (const unsigned short*)X, ldx,
+                       w, pred, y, (unsigned short*)dX);
+    VX_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_head_bwd_sums, dim3((K + 63) / 64), dim3(1024), 0, st, M, K, (const unsigned short*)X,
+                       ldx, pred, y, dw, db, loss);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
 
 extern "C" int vx_colsum_bf16(int M, int N, const void* G, int ldg, float* out, vx_stream stream) {
     VX_REQUIRE(G && out && M > 0 && N > 0, VX_ERR_ARG, "bad arguments");
-    hipLaunchKernelGGL(k_colsum_bf16, dim3((N + 255) / 256, 64), dim3(256), 0, (hipStream_t)stream, M, N,
+    hipLaunchKernelGGL(k_colsum_bf16, dim3((N + 63) / 64), dim3(1024), 0, (hipStream_t)stream, M, N,
                        (const unsigned short*)G, ldg, out);
     VX_LAUNCH_CHECK();
     return VX_OK;
@@ -343,12 +408,12 @@ extern "C" int vx_cast_bf16(int n, const float* in, void* out, vx_stream stream)
 extern "C" int vx_mlp_descriptors(int n, const double* data, const float* vec, float* out32, vx_stream stream) {
     VX_REQUIRE(data && vec && out32 && n > 0, VX_ERR_ARG, "bad arguments");
     hipStream_t st = (hipStream_t)stream;
+    constexpr int nblk = 256;   // every workgroup writes its 29 partials: no clearing needed
     double* sums = nullptr;
-    VX_HIP(hipMallocAsync((void**)&sums, 32 * sizeof(double), st));
-    VX_HIP(hipMemsetAsync(sums, 0, 32 * sizeof(double), st));
-    hipLaunchKernelGGL(k_moment_sums, dim3(256), dim3(256), 0, st, n, data, vec, sums);
+    VX_HIP(hipMallocAsync((void**)&sums, nblk * 32 * sizeof(double), st));
+    hipLaunchKernelGGL(k_moment_sums, dim3(nblk), dim3(256), 0, st, n, data, vec, sums);
     VX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_moment_finish, dim3(1), dim3(64), 0, st, n, sums, out32);
+    hipLaunchKernelGGL(k_moment_finish, dim3(1), dim3(64), 0, st, n, nblk, sums, out32);
     VX_LAUNCH_CHECK();
     VX_HIP(hipFreeAsync(sums, st));
     return VX_OK;
