@@ -532,6 +532,30 @@ def test_xpass_variants_identical(gpu, n, mode):
         assert np.array_equal(b, res[0][0]) and np.array_equal(t, res[0][1])
 
 
+def test_xpass_variants_identical_on_pencil_grid(gpu):
+    """N = 270, where the slab exceeds LDS and the pencil passes run: the
+    default x pass (register-prefetch, 4-column tiles), the LDS x pass and
+    2-column register tiles give bit-identical scores and translations."""
+    n = 270
+    rng = np.random.default_rng(1270)
+    td, tv = _maps(rng, n, 0.3)
+    sd, sv = _maps(rng, n, 0.3)
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (4, 3)))).cuda()
+    res = []
+    for variant, tile in ((None, None), (1, 4), (3, 2)):
+        _say("x-pass variant on a pencil grid", n, variant, tile)
+        plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=2)
+        if variant is not None:
+            plan.tune(xpass_variant=variant, xpass_tile=tile)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), None, 0.3)
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), None, 0.3)
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+        plan.close()
+    for b, t in res[1:]:
+        assert np.array_equal(b, res[0][0]) and np.array_equal(t, res[0][1])
+
+
 def test_correlate_through_pencils_n250(gpu):
     """vx_correlate at N = 250, where the slab exceeds LDS and the pencil
     passes run: one channel's correlation map within 1e-4 of the FP64

@@ -1443,8 +1443,15 @@ k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G, int 
     }
 }
 
+// Values per pencil workgroup: ~2k complex (32 KB of ping-pong LDS, four to
+// five 256-thread workgroups per CU).  At ~4k (two workgroups per CU) the
+// pencil searches ran 8-39 % slower (N = 208..343): too few waves to hide the
+// LDS latency of the stages; at ~1k the shorter lines per workgroup lost more
+// than the extra waves gained.
+constexpr int kPencilValues = 2048;
+
 static int pencil_pairs(int n) {
-    int P = 4096 / n;          // ~4k complex values per workgroup
+    int P = kPencilValues / n;
     if (P < 1) P = 1;
     if (P > 32) P = 32;
     return P;
@@ -1452,9 +1459,9 @@ static int pencil_pairs(int n) {
 
 static int pencil_T(int n) {
     // columns per y-pass workgroup: a power of two (runs of 8 T bytes stay
-    // aligned to 64-byte segments), ~4k complex values, at most 16
+    // aligned to segments), at most 16
     int T = 16;
-    while (T > 1 && T * n > 4096) T >>= 1;
+    while (T > 1 && T * n > kPencilValues) T >>= 1;
     return T;
 }
 
@@ -1776,7 +1783,7 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
                  int xvariant = 0, int xT = 0, bool use_static = true) {
     const int nhz = n / 2 + 1;
     if (tgt != nullptr && xvariant == 3) {
-        const int T = xT > 0 ? xT : 8;
+        const int T = xT > 0 ? xT : (n * 8 <= 256 * kXcorrPF ? 8 : 4);
         VX_REQUIRE(n * T <= 256 * kXcorrPF, VX_ERR_UNSUPPORTED,
                    "x-pass variant 3 needs grid edge x tile <= %d", 256 * kXcorrPF);
         const size_t lds = ((size_t)2 * n * T + fd.ntw) * sizeof(float2);
@@ -1952,9 +1959,10 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     memset(p, 0, sizeof(*p));
     p->n = n; p->nhz = nhz; p->mode = mode; p->batch = batch; p->flags = flags;
     p->lo = 0; p->w = n; p->q_dirty = true;
-    // register-accumulating prefetch x pass with 8-column tiles where it
-    // applies (the autotuner's usual pick on MI355X), else the LDS one
-    p->xvariant = n * 8 <= 256 * kXcorrPF ? 3 : 1;
+    // register-accumulating prefetch x pass (the autotuner's usual pick on
+    // MI355X) with 8-column tiles, 4-column above grid edge 256 (4-6 % faster
+    // than the LDS x pass there, N = 300..360), else the LDS one
+    p->xvariant = n * 4 <= 256 * kXcorrPF ? 3 : 1;
     p->xT = 0;
     p->nstreams = 1;
     p->static_plans = 1;
