@@ -461,13 +461,15 @@ def test_slab_shapes_identical_at_n175(gpu):
                                     (91, nat.VX_MODE_VEC), (162, nat.VX_MODE_PROB),
                                     (24, nat.VX_MODE_CC), (45, nat.VX_MODE_VEC), (64, nat.VX_MODE_PROB),
                                     (80, nat.VX_MODE_VEC), (77, nat.VX_MODE_VEC),
-                                    (200, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (250, nat.VX_MODE_VEC)])
+                                    (200, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (250, nat.VX_MODE_VEC),
+                                    (270, nat.VX_MODE_VEC), (308, nat.VX_MODE_PROB)])
 def test_static_plans_identical(gpu, n, mode):
     """Grid edges with compile-time stage plans (175 = 5 5 7, 128 = 8 8 2,
     96 = 8 4 3, 104 = 8 13, 110 = 2 5 11, 91 = 7 13 with a runtime x pass,
     162 = 2 3 3 3 3; in the 8 x 512 slab shape 24, 45, 64, 80, 77 = 7 11;
     in the 24 x 1024 shape 200 = 8 5 5, 182 = 2 7 13; 250 = 2 5 5 5 on the
-    pencils with a compile-time x pass):
+    pencils with a compile-time x pass in 8-column tiles, 270 = 2 3 3 3 5
+    and 308 = 4 7 11 in 4-column tiles):
     the slab and x-pass kernels built on them give the
     runtime-descriptor kernels' scores and translations, in the one-group
     modes and in prob mode (two groups, mixed score).  The FFT code may
@@ -493,7 +495,7 @@ def test_static_plans_identical(gpu, n, mode):
     for static in (True, False):
         _say("static plans", n, mode, static)
         plan = engine.SearchPlan(n, mode, batch=2)
-        plan.tune(static_plans=static, xpass_variant=3, xpass_tile=8)
+        plan.tune(static_plans=static, xpass_variant=3, xpass_tile=8 if n <= 256 else 4)
         plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), prob[0] if prob else None, 0.3)
         plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), prob[1] if prob else None, 0.3)
         if prob:

@@ -115,14 +115,16 @@ def _fftw_fast(n):
 
 
 @pytest.mark.parametrize("name,lo,hi", [("StaticLengths", 90, 180), ("SmallLengths", 16, 89),
-                                         ("MediumLengths", 181, 200), ("TileLengths", 201, 256)])
+                                         ("MediumLengths", 181, 200), ("TileLengths", 201, 256),
+                                         ("LargeLengths", 257, 400)])
 def test_static_lengths_follow_the_descriptor_order(name, lo, hi):
     """Every compile-time plan lists its radices in fft_desc_create's order
     (else the host check would silently fall back to the runtime plan); the
     16 x 1024-slab list is every FFTW-fast length from 90 to 180, the
     8 x 512 one every such length from 16 to 89 with two stages or more, the
-    24 x 1024 one every such length from 181 to 200, the x-pass-only one
-    every such length from 201 to 256."""
+    24 x 1024 one every such length from 181 to 200, the x-pass-only ones
+    every such length from 201 to 256 (8-column tiles) and from 257 to 400
+    (4-column tiles)."""
     src = open(os.path.join(ROOT, "vesper_python_amd", "csrc", "vx_search.hip")).read()
     m = re.search(r"using %s = PlanList<(.*?)>;" % name, src, re.S)
     assert m

@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <vector>
 #include <new>
+#include <type_traits>
 
 #include "vx_common.hpp"
 #include "vx_fft.hpp"
@@ -96,6 +97,18 @@ using TileLengths = PlanList<StaticPlan<8, 2, 13>, StaticPlan<2, 3, 5, 7>, Stati
     StaticPlan<4, 5, 11>, StaticPlan<8, 4, 7>, StaticPlan<3, 3, 5, 5>, StaticPlan<3, 7, 11>,
     StaticPlan<2, 3, 3, 13>, StaticPlan<8, 2, 3, 5>, StaticPlan<3, 3, 3, 3, 3>, StaticPlan<5, 7, 7>,
     StaticPlan<2, 5, 5, 5>, StaticPlan<4, 3, 3, 7>, StaticPlan<8, 8, 4>>;
+
+// The FFTW-fast lengths from 257 to 400 (the larger grids of uneven map
+// pairs): the register x pass takes them in 4-column tiles (N x 4 <= 2048)
+// with a compile-time tile plan.
+using LargeLengths = PlanList<StaticPlan<4, 5, 13>, StaticPlan<8, 3, 11>, StaticPlan<2, 3, 3, 3, 5>,
+    StaticPlan<3, 7, 13>, StaticPlan<5, 5, 11>, StaticPlan<8, 5, 7>, StaticPlan<8, 4, 3, 3>, StaticPlan<2, 3, 7, 7>,
+    StaticPlan<3, 3, 3, 11>, StaticPlan<4, 3, 5, 5>, StaticPlan<4, 7, 11>, StaticPlan<8, 3, 13>,
+    StaticPlan<3, 3, 5, 7>, StaticPlan<8, 8, 5>, StaticPlan<4, 3, 3, 3, 3>, StaticPlan<5, 5, 13>,
+    StaticPlan<2, 3, 5, 11>, StaticPlan<8, 2, 3, 7>, StaticPlan<7, 7, 7>, StaticPlan<2, 5, 5, 7>,
+    StaticPlan<3, 3, 3, 13>, StaticPlan<8, 4, 11>, StaticPlan<8, 3, 3, 5>, StaticPlan<4, 7, 13>,
+    StaticPlan<3, 5, 5, 5>, StaticPlan<2, 3, 3, 3, 7>, StaticPlan<8, 8, 2, 3>, StaticPlan<5, 7, 11>,
+    StaticPlan<2, 3, 5, 13>, StaticPlan<8, 7, 7>, StaticPlan<4, 3, 3, 11>, StaticPlan<8, 2, 5, 5>>;
 
 // Calls f(SP{}) for the listed plan whose length is n; false if none is.
 template <class F, class... Ps>
@@ -807,15 +820,19 @@ k_xcorr_lds_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 // 5 waves per SIMD (96 VGPRs) hold without spills when the tile values per
 // thread are at most 6 and the radices are small: radix 11 or 13 stages, or
 // radix 7 and 8 together, need more registers and keep the compiler's choice.
-constexpr int xcorr_waves_hint(int PF, unsigned mask) {
+// The one-group 4-column compile-time tiles of 343 = 7 7 7, 350 = 2 5 5 7 and
+// 378 = 2 3 3 3 7 spill 12-36 VGPRs at that budget and keep the compiler's
+// choice too.
+constexpr int xcorr_waves_hint(int PF, unsigned mask, int G = 0, int n = 0, int nl = 0) {
     return (PF <= 6 && (mask & (radix_bit(11) | radix_bit(13))) == 0 &&
-            (mask & (radix_bit(7) | radix_bit(8))) != (radix_bit(7) | radix_bit(8)))
+            (mask & (radix_bit(7) | radix_bit(8))) != (radix_bit(7) | radix_bit(8)) &&
+            !(G == 1 && nl == 4 && (n == 343 || n == 350 || n == 378)))
                ? 5
                : 1;
 }
 
 template <int G, int PF, unsigned RM, int NT, class SP = DynamicTile>
-__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(xcorr_waves_hint(PF, SP::kStatic ? SP::kMask : RM))))
+__global__ void __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(xcorr_waves_hint(PF, SP::kStatic ? SP::kMask : RM, G, SP::kN, SP::kNL))))
 k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2* __restrict__ out,
                FftDesc fdg, int n_, int nhz_, int nb, int C, int T_, int ntiles, unsigned grp_mask, int lo, int w,
                int tile0) {
@@ -1753,23 +1770,31 @@ template <int G, unsigned RM>
 static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* spec, const float2* tgt,
                           float2* out, const FftDesc& fd, int n, int nhz, int nb, int C, int T, int ntiles,
                           unsigned grp_mask, int lo, int w, bool use_static) {
-    // compile-time tile plans: the full 8-column tiles, with the same threads
-    // and values per thread as the runtime form; a last partial tile (when
-    // n (n / 2 + 1) is not a multiple of 8) runs the runtime form
-    const int full = n * nhz / 8;
-    auto launch_static = [&](auto sp) {
+    // compile-time tile plans: the full 8-column tiles (4-column above grid
+    // edge 256), with the same threads and values per thread as the runtime
+    // form; a last partial tile (when n (n / 2 + 1) is not a multiple of the
+    // tile width) runs the runtime form
+    const int full = n * nhz / T;
+    auto launch_static = [&](auto sp, auto width) {
         using SP = decltype(sp);
-        constexpr int E = SP::kN * 8;
+        constexpr int TW = decltype(width)::value;
+        using TL = typename SP::template Tile<TW>;
+        constexpr int E = SP::kN * TW;
         constexpr int NT = E <= 64 * kXcorrPF ? 64 : (E <= 128 * kXcorrPF ? 128 : 256);
         constexpr int PF = (NT == 256 && E <= 256 * 6) ? 6 : kXcorrPF;
         static_assert(E <= NT * PF, "tile exceeds the workgroup's values");
-        launch_reg_pf_nt<G, SP::kMask, NT, PF, typename SP::template Tile<8>>(xcorr_grid(full, nb), lds, st, spec,
-                                                                             tgt, out, fd, n, nhz, nb, C, T, full,
-                                                                             grp_mask, lo, w, 0);
+        launch_reg_pf_nt<G, SP::kMask, NT, PF, TL>(xcorr_grid(full, nb), lds, st, spec, tgt, out, fd, n, nhz, nb, C,
+                                                   T, full, grp_mask, lo, w, 0);
     };
-    if (use_static && T == 8 && full > 0 &&
-        (with_static_plan(n, StaticLengths{}, launch_static) || with_static_plan(n, SmallLengths{}, launch_static) ||
-         with_static_plan(n, MediumLengths{}, launch_static) || with_static_plan(n, TileLengths{}, launch_static))) {
+    auto tiles8 = [&](auto sp) { launch_static(sp, std::integral_constant<int, 8>{}); };
+    auto tiles4 = [&](auto sp) { launch_static(sp, std::integral_constant<int, 4>{}); };
+    bool hit = false;
+    if (use_static && full > 0 && T == 8)
+        hit = with_static_plan(n, StaticLengths{}, tiles8) || with_static_plan(n, SmallLengths{}, tiles8) ||
+              with_static_plan(n, MediumLengths{}, tiles8) || with_static_plan(n, TileLengths{}, tiles8);
+    else if (use_static && full > 0 && T == 4 && n > 256)
+        hit = with_static_plan(n, LargeLengths{}, tiles4);
+    if (hit) {
         if (full < ntiles)
             launch_reg_pf_dyn<G, RM>(xcorr_grid(ntiles - full, nb), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T,
                                      ntiles - full, grp_mask, lo, w, full);
