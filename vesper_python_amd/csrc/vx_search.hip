@@ -1339,7 +1339,8 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
 
 // Forward r2c along z for a chunk of P line pairs of one x-slab: two real
 // rows packed into one complex line, FFT, unpacked into half spectra.
-// Only slabs x in [lo, lo + w) and pairs covering rows [lo, lo + w).
+// Only slabs x in [lo, lo + w), pairs covering rows [lo, lo + w) and loads of
+// z in [lo, lo + w).
 __global__ void __launch_bounds__(256)
 k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w, int P) {
     extern __shared__ float2 lds[];
@@ -1356,8 +1357,10 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
     const float ninv = 1.0f / (float)n;
     for (int e = threadIdx.x; e < np * n; e += 256) {
         const int ll = fdiv(e, n, ninv), z = e - ll * n, l = l0 + ll;
-        const float re = src[(size_t)(2 * l) * n + z];
-        const float im = (2 * l + 1 < n) ? src[(size_t)(2 * l + 1) * n + z] : 0.f;
+        // the query planes are zero outside the support box: no loads there
+        const bool in = z >= lo && z < lo + w;
+        const float re = in ? src[(size_t)(2 * l) * n + z] : 0.f;
+        const float im = (in && 2 * l + 1 < n) ? src[(size_t)(2 * l + 1) * n + z] : 0.f;
         A[e] = make_float2(re, im);
     }
     __syncthreads();
