@@ -1379,16 +1379,27 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
 // In-place complex FFT along y of a tile of T kz-columns of one x-slab.
 // Forward: rows outside [ylo, ylo + yw) are read as zero.  Inverse
 // (unnormalised): conj on load and store.
+// One workgroup per (column tile, x-slab, plane), nxs x-slabs, on a 1-D grid
+// mapped XCD-aware: workgroup ids reach the 8 XCDs round-robin, and XCD k
+// takes a contiguous range of (tile, slab, plane), so the column tiles that
+// share a row's cache lines (T columns are 8 T bytes of a row) read them
+// through one L2.
 __global__ void __launch_bounds__(256)
-k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n, int nhz, int T, int inverse, int ylo, int yw, int xlo) {
+k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n, int nhz, int T, int inverse, int ylo, int yw, int xlo,
+         int ntile, int nxs, int total) {
     extern __shared__ float2 lds[];
+    const int per = (total + 7) >> 3;
+    const int m = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if ((int)(blockIdx.x >> 3) >= per || m >= total) return;
+    const int tile = m % ntile, rest = m / ntile;
+    const int xs = rest % nxs, plane = rest / nxs;
     float2* A = lds;
     float2* Bf = lds + n * T;
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
-    const int k0 = blockIdx.x * T;
+    const int k0 = tile * T;
     const int nt = min(T, nhz - k0);
-    const int x = xlo + blockIdx.y;
-    float2* base = buf + ((size_t)blockIdx.z * n + x) * (size_t)n * nhz + k0;
+    const int x = xlo + xs;
+    float2* base = buf + ((size_t)plane * n + x) * (size_t)n * nhz + k0;
     const float ntinv = 1.0f / (float)nt;
     for (int e = threadIdx.x; e < n * nt; e += 256) {
         const int y = fdiv(e, nt, ntinv), t = e - y * nt;
@@ -1506,8 +1517,11 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
     const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
     VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
     allow_lds_any(k_axis_y, ly);
-    hipLaunchKernelGGL(k_axis_y, dim3((nhz + T - 1) / T, w, nplanes), dim3(256), ly, st, spec, fd, n, nhz, T, 0,
-                       2 * l0, 2 * (l1 - l0), lo);
+    {
+        const int ntile = (nhz + T - 1) / T, total = ntile * w * nplanes;
+        hipLaunchKernelGGL(k_axis_y, dim3((unsigned)(((total + 7) / 8) * 8)), dim3(256), ly, st, spec, fd, n, nhz, T,
+                           0, 2 * l0, 2 * (l1 - l0), lo, ntile, w, total);
+    }
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -1519,8 +1533,11 @@ int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, uns
     const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
     VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
     allow_lds_any(k_axis_y, ly);
-    hipLaunchKernelGGL(k_axis_y, dim3((nhz + T - 1) / T, n, nbg), dim3(256), ly, st, sum, fd, n, nhz, T, 1,
-                       0, n, 0);
+    {
+        const int ntile = (nhz + T - 1) / T, total = ntile * n * nbg;
+        hipLaunchKernelGGL(k_axis_y, dim3((unsigned)(((total + 7) / 8) * 8)), dim3(256), ly, st, sum, fd, n, nhz, T,
+                           1, 0, n, 0, ntile, n, total);
+    }
     VX_LAUNCH_CHECK();
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
