@@ -1379,38 +1379,45 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
 // In-place complex FFT along y of a tile of T kz-columns of one x-slab.
 // Forward: rows outside [ylo, ylo + yw) are read as zero.  Inverse
 // (unnormalised): conj on load and store.
-// One workgroup per (column tile, x-slab, plane), nxs x-slabs, on a 1-D grid
-// mapped XCD-aware: workgroup ids reach the 8 XCDs round-robin, and XCD k
-// takes a contiguous range of (tile, slab, plane), so the column tiles that
-// share a row's cache lines (T columns are 8 T bytes of a row) read them
-// through one L2.
+// One workgroup per (column tile, x-slab, plane) over tiles [tile0, tile0 +
+// ntile) and nxs x-slabs, on a 1-D grid mapped XCD-aware: workgroup ids reach
+// the 8 XCDs round-robin, and XCD k takes a contiguous range of (tile, slab,
+// plane), so the column tiles that share a row's cache lines (T columns are
+// 8 T bytes of a row) read them through one L2.  With a compile-time tile
+// plan (SP = StaticTile<T, radices>) every tile of the launch is full.
+template <class SP = DynamicTile>
 __global__ void __launch_bounds__(256)
-k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n, int nhz, int T, int inverse, int ylo, int yw, int xlo,
-         int ntile, int nxs, int total) {
+k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n_, int nhz_, int T_, int inverse, int ylo, int yw, int xlo,
+         int ntile, int nxs, int total, int tile0) {
     extern __shared__ float2 lds[];
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
+    const int T = SP::kStatic ? SP::kNL : T_;
     const int per = (total + 7) >> 3;
     const int m = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if ((int)(blockIdx.x >> 3) >= per || m >= total) return;
-    const int tile = m % ntile, rest = m / ntile;
+    const int tile = m % ntile + tile0, rest = m / ntile;
     const int xs = rest % nxs, plane = rest / nxs;
     float2* A = lds;
     float2* Bf = lds + n * T;
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * T);
     const int k0 = tile * T;
-    const int nt = min(T, nhz - k0);
+    const int nt = SP::kStatic ? T : min(T, nhz - k0);
     const int x = xlo + xs;
     float2* base = buf + ((size_t)plane * n + x) * (size_t)n * nhz + k0;
     const float ntinv = 1.0f / (float)nt;
     for (int e = threadIdx.x; e < n * nt; e += 256) {
-        const int y = fdiv(e, nt, ntinv), t = e - y * nt;
+        const int y = qdiv<SP::kStatic>(e, nt, ntinv), t = e - y * nt;
         float2 v = make_float2(0.f, 0.f);
         if (y >= ylo && y < ylo + yw) v = base[(size_t)y * nhz + t];
         A[e] = inverse ? cconj(v) : v;
     }
     __syncthreads();
-    const float2* r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
+    const float2* r;
+    if constexpr (SP::kStatic) r = SP::template tile_fft<256>(A, Bf, fd.twid);
+    else r = fft_lds_pp(A, Bf, fd, nt, nt, 1, true);
     for (int e = threadIdx.x; e < n * nt; e += 256) {
-        const int y = fdiv(e, nt, ntinv), t = e - y * nt;
+        const int y = qdiv<SP::kStatic>(e, nt, ntinv), t = e - y * nt;
         base[(size_t)y * nhz + t] = inverse ? cconj(r[e]) : r[e];
     }
 }
@@ -1502,8 +1509,44 @@ static int allow_lds_any(K kernel, size_t bytes) {
     return VX_OK;
 }
 
+// The y pass over nxs x-slabs (from xlo) of nplanes planes: full column tiles
+// on a compile-time tile plan where the grid edge has one (8-column tiles for
+// 201..256, 4-column above), a last partial tile and the other edges on the
+// runtime plan.
+static int launch_axis_y(float2* buf, const FftDesc& fd, int n, int inverse, int ylo, int yw, int xlo, int nxs,
+                         int nplanes, bool use_static, hipStream_t st) {
+    const int nhz = n / 2 + 1;
+    const int T = pencil_T(n);
+    const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
+    VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
+    const int ntile = (nhz + T - 1) / T, full = nhz / T;
+    auto grid = [](int total) { return dim3((unsigned)(((total + 7) / 8) * 8)); };
+    auto go = [&](auto sp, auto width) {
+        using TL = typename decltype(sp)::template Tile<decltype(width)::value>;
+        const int total = full * nxs * nplanes;
+        allow_lds_any(k_axis_y<TL>, ly);
+        hipLaunchKernelGGL(k_axis_y<TL>, grid(total), dim3(256), ly, st, buf, fd, n, nhz, T, inverse, ylo, yw, xlo,
+                           full, nxs, total, 0);
+    };
+    bool hit = false;
+    if (use_static && full > 0 && T == 8)
+        hit = with_static_plan(n, TileLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 8>{}); });
+    else if (use_static && full > 0 && T == 4)
+        hit = with_static_plan(n, LargeLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 4>{}); });
+    VX_LAUNCH_CHECK();
+    const int t0 = hit ? full : 0;
+    if (t0 < ntile) {
+        const int nt = ntile - t0, total = nt * nxs * nplanes;
+        allow_lds_any(k_axis_y<DynamicTile>, ly);
+        hipLaunchKernelGGL(k_axis_y<DynamicTile>, grid(total), dim3(256), ly, st, buf, fd, n, nhz, T, inverse, ylo,
+                           yw, xlo, nt, nxs, total, t0);
+        VX_LAUNCH_CHECK();
+    }
+    return VX_OK;
+}
+
 int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
-                      hipStream_t st) {
+                      hipStream_t st, bool use_static = true) {
     const int nhz = n / 2 + 1;
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
@@ -1513,32 +1556,14 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
     hipLaunchKernelGGL(k_fwd_z, dim3((l1 - l0 + P - 1) / P, w, nplanes), dim3(256), lz, st, q, spec, fd, n, nhz,
                        lo, w, P);
     VX_LAUNCH_CHECK();
-    const int T = pencil_T(n);
-    const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
-    VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
-    allow_lds_any(k_axis_y, ly);
-    {
-        const int ntile = (nhz + T - 1) / T, total = ntile * w * nplanes;
-        hipLaunchKernelGGL(k_axis_y, dim3((unsigned)(((total + 7) / 8) * 8)), dim3(256), ly, st, spec, fd, n, nhz, T,
-                           0, 2 * l0, 2 * (l1 - l0), lo, ntile, w, total);
-    }
-    VX_LAUNCH_CHECK();
-    return VX_OK;
+    return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st);
 }
 
 int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys, int nkeys,
-                      float* real_out, hipStream_t st) {
+                      float* real_out, hipStream_t st, bool use_static = true) {
     const int nhz = n / 2 + 1;
-    const int T = pencil_T(n);
-    const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
-    VX_REQUIRE(ly <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
-    allow_lds_any(k_axis_y, ly);
-    {
-        const int ntile = (nhz + T - 1) / T, total = ntile * n * nbg;
-        hipLaunchKernelGGL(k_axis_y, dim3((unsigned)(((total + 7) / 8) * 8)), dim3(256), ly, st, sum, fd, n, nhz, T,
-                           1, 0, n, 0, ntile, n, total);
-    }
-    VX_LAUNCH_CHECK();
+    int rc = launch_axis_y(sum, fd, n, 1, 0, n, 0, n, nbg, use_static, st);
+    if (rc != VX_OK) return rc;
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
@@ -2185,7 +2210,7 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     VX_LAUNCH_CHECK();
     // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
     // spec holds B*C planes >= C planes.
-    int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st)
+    int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st, p->static_plans != 0)
                        : launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st, 0, p->static_plans != 0);
     if (rc != VX_OK) return rc;
     // treat the C channels as "batch" 1 with C channels, forward-conj mode
@@ -2333,7 +2358,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
         VX_LAUNCH_CHECK();
     }
     VX_MARK(1);
-    rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st)
+    rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->static_plans != 0)
                    : launch_fwd_zy(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt,
                                    p->static_plans != 0);
     if (rc != VX_OK) return rc;
@@ -2351,7 +2376,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     if (rc != VX_OK) return rc;
     VX_MARK(3);
     float* ro = (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
-    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st)
+    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0)
                    : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt,
                                    p->static_plans != 0);
     if (rc != VX_OK) return rc;
@@ -2442,13 +2467,13 @@ extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float*
     const int xv = p->xvariant == 0 ? 0 : 1;   // the FFT x passes valid for every grid edge
     for (int c = 0; c < nch; ++c) {
         const float* qc = q + (size_t)c * n3;
-        int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st)
+        int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st, p->static_plans != 0)
                            : launch_fwd_zy(qc, p->spec, p->fd, n, 1, 0, n, st, 0, p->static_plans != 0);
         if (rc != VX_OK) return rc;
         rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT,
                           p->static_plans != 0);
         if (rc != VX_OK) return rc;
-        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st)
+        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, p->static_plans != 0)
                        : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, 0,
                                        p->static_plans != 0);
         if (rc != VX_OK) return rc;
