@@ -1341,9 +1341,16 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
 // rows packed into one complex line, FFT, unpacked into half spectra.
 // Only slabs x in [lo, lo + w), pairs covering rows [lo, lo + w) and loads of
 // z in [lo, lo + w).
+// With a compile-time plan (SP = StaticPlan<radices>) the lines transform in
+// place on registers (at most kPencilMaxV values per thread).
+constexpr int kPencilMaxV = 8;
+
+template <class SP = DynamicPlan>
 __global__ void __launch_bounds__(256)
-k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n, int nhz, int lo, int w, int P) {
+k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int n_, int nhz_, int lo, int w, int P) {
     extern __shared__ float2 lds[];
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     float2* A = lds;
     float2* Bf = lds + P * n;
     const FftDesc fd = stage_twiddles(fdg, lds + 2 * P * n);
@@ -1356,7 +1363,7 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
     const float* src = q + (plane * n + x) * (size_t)n * n;
     const float ninv = 1.0f / (float)n;
     for (int e = threadIdx.x; e < np * n; e += 256) {
-        const int ll = fdiv(e, n, ninv), z = e - ll * n, l = l0 + ll;
+        const int ll = qdiv<SP::kStatic>(e, n, ninv), z = e - ll * n, l = l0 + ll;
         // the query planes are zero outside the support box: no loads there
         const bool in = z >= lo && z < lo + w;
         const float re = in ? src[(size_t)(2 * l) * n + z] : 0.f;
@@ -1364,11 +1371,13 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
         A[e] = make_float2(re, im);
     }
     __syncthreads();
-    const float2* r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
+    const float2* r = A;
+    if constexpr (SP::kStatic) SP::template lines_inplace<256, kPencilMaxV>(A, fd.twid, np);
+    else r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
     float2* out = spec + (plane * n + x) * (size_t)n * nhz;
     const float hinv = 1.0f / (float)nhz;
     for (int u = threadIdx.x; u < np * nhz; u += 256) {
-        const int ll = fdiv(u, nhz, hinv), k = u - ll * nhz, l = l0 + ll;
+        const int ll = qdiv<SP::kStatic>(u, nhz, hinv), k = u - ll * nhz, l = l0 + ll;
         const float2 zk = r[ll * n + k];
         const float2 wv = r[ll * n + (k == 0 ? 0 : n - k)];
         out[(size_t)(2 * l) * nhz + k] = make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
@@ -1424,10 +1433,13 @@ k_axis_y(float2* __restrict__ buf, FftDesc fdg, int n_, int nhz_, int T_, int in
 
 // Inverse c2r along z for a chunk of P line pairs (rows 2l, 2l+1 of the
 // y-inverted half spectrum), then max + first argmax -> atomicMax.
+template <class SP = DynamicPlan>
 __global__ void __launch_bounds__(256)
-k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G, int P,
+k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G, int P,
         unsigned long long* __restrict__ keys, int nkeys, float* __restrict__ real_out) {
     extern __shared__ float2 lds[];
+    const int n = SP::kStatic ? SP::kN : n_;
+    const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     __shared__ float s_v[16];
     __shared__ uint32_t s_i[16];
     float2* A = lds;
@@ -1443,7 +1455,7 @@ k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G, int 
     const bool even = (n & 1) == 0;
     const float ninv = 1.0f / (float)n;
     for (int u = threadIdx.x; u < np * n; u += 256) {
-        const int ll = fdiv(u, n, ninv), k = u - ll * n, l = l0 + ll;
+        const int ll = qdiv<SP::kStatic>(u, n, ninv), k = u - ll * n, l = l0 + ll;
         const bool hasb = (2 * l + 1) < n;
         const int kk = (k <= half) ? k : n - k;
         float2 a = src[(size_t)(2 * l) * nhz + kk];
@@ -1458,12 +1470,14 @@ k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n, int nhz, int G, int 
         A[u] = zc;
     }
     __syncthreads();
-    const float2* r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
+    const float2* r = A;
+    if constexpr (SP::kStatic) SP::template lines_inplace<256, kPencilMaxV>(A, fd.twid, np);
+    else r = fft_lds_pp(A, Bf, fd, np, 1, n, false);
     float bv = -INFINITY;
     uint32_t bi = 0xffffffffu;
     const size_t n3 = (size_t)n * n * n;
     for (int e = threadIdx.x; e < np * n; e += 256) {
-        const int ll = fdiv(e, n, ninv), z = e - ll * n, l = l0 + ll;
+        const int ll = qdiv<SP::kStatic>(e, n, ninv), z = e - ll * n, l = l0 + ll;
         const float2 f = r[e];
         const uint32_t ia = ((uint32_t)x * n + 2 * l) * n + z;
         if (better(f.x, ia, bv, bi)) { bv = f.x; bi = ia; }
@@ -1552,9 +1566,15 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
     const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
-    allow_lds_any(k_fwd_z, lz);
-    hipLaunchKernelGGL(k_fwd_z, dim3((l1 - l0 + P - 1) / P, w, nplanes), dim3(256), lz, st, q, spec, fd, n, nhz,
-                       lo, w, P);
+    const dim3 gz((l1 - l0 + P - 1) / P, w, nplanes);
+    auto zgo = [&](auto sp) {
+        using SP = decltype(sp);
+        allow_lds_any(k_fwd_z<SP>, lz);
+        hipLaunchKernelGGL(k_fwd_z<SP>, gz, dim3(256), lz, st, q, spec, fd, n, nhz, lo, w, P);
+    };
+    if (!(use_static && n * P <= 256 * kPencilMaxV &&
+          (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
+        zgo(DynamicPlan{});
     VX_LAUNCH_CHECK();
     return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st);
 }
@@ -1567,10 +1587,16 @@ int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, uns
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
-    allow_lds_any(k_inv_z, lz);
     const int npair = (n + 1) / 2;
-    hipLaunchKernelGGL(k_inv_z, dim3((npair + P - 1) / P, n, nbg), dim3(256), lz, st, sum, fd, n, nhz, G, P, keys,
-                       nkeys, real_out);
+    const dim3 gz((npair + P - 1) / P, n, nbg);
+    auto zgo = [&](auto sp) {
+        using SP = decltype(sp);
+        allow_lds_any(k_inv_z<SP>, lz);
+        hipLaunchKernelGGL(k_inv_z<SP>, gz, dim3(256), lz, st, sum, fd, n, nhz, G, P, keys, nkeys, real_out);
+    };
+    if (!(use_static && n * P <= 256 * kPencilMaxV &&
+          (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
+        zgo(DynamicPlan{});
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
