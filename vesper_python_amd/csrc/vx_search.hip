@@ -206,8 +206,12 @@ __device__ __forceinline__ float rot_vec(float v0, float v1, float v2, const flo
 //   KIND_PROB:    svec as above + sprob {p1..p4} each masked by its own != 0
 //                                                     -> out[0..6]
 //   KIND_DATA64:  sdat (float64, masked by != 0)      -> outd
+// svec / sscal / sprob hold only the support box [lo, lo + w)^3 of the source
+// (x-major, edge w): every voxel with non-zero density or probability lies
+// inside it, so a rotated position outside reads nothing, as a zero voxel
+// would.  sdat holds the whole grid.
 template <int KIND>
-__device__ __forceinline__ void gather_voxel(int x, int y, int z, int n, const float* F, bool fma,
+__device__ __forceinline__ void gather_voxel(int x, int y, int z, int n, int lo, int w, const float* F, bool fma,
                                              const float4* __restrict__ svec,
                                              const float2* __restrict__ sscal,
                                              const float4* __restrict__ sprob,
@@ -233,7 +237,14 @@ __device__ __forceinline__ void gather_voxel(int x, int y, int z, int n, const f
         const float o1 = rot_coord(n0, n1, n2, F + 3, cent, fma);
         const float o2 = rot_coord(n0, n1, n2, F + 6, cent, fma);
         if (!(o0 >= 0.f && o1 >= 0.f && o2 >= 0.f && o0 < fn && o1 < fn && o2 < fn)) continue;
-        const size_t si = ((size_t)(int)o0 * n + (int)o1) * n + (int)o2;
+        size_t si;
+        if (KIND == KIND_DATA64) {
+            si = ((size_t)(int)o0 * n + (int)o1) * n + (int)o2;
+        } else {
+            const int b0 = (int)o0 - lo, b1 = (int)o1 - lo, b2 = (int)o2 - lo;
+            if ((unsigned)b0 >= (unsigned)w || (unsigned)b1 >= (unsigned)w || (unsigned)b2 >= (unsigned)w) continue;
+            si = ((size_t)b0 * w + b1) * w + b2;
+        }
         if (KIND == KIND_VEC || KIND == KIND_PROB) {
             if (!done_v) {
                 const float4 s = svec[si];
@@ -270,7 +281,7 @@ k_gather(const float4* __restrict__ svec, const float2* __restrict__ sscal,
          const float4* __restrict__ sprob, const double* __restrict__ sdat,
          const float* __restrict__ rot, int n, int nzb, int fma,
          float* __restrict__ q, double* __restrict__ qd,
-         unsigned long long* __restrict__ keys, int nkeys, int lo, int w) {
+         unsigned long long* __restrict__ keys, int nkeys, int lo, int w, int slo, int sw) {
     const int b = blockIdx.z;
     if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x < nkeys) keys[(size_t)b * nkeys + threadIdx.x] = 0ull;
     // destination box [lo, lo + w)^3 holds every voxel that can receive a
@@ -285,7 +296,7 @@ k_gather(const float4* __restrict__ svec, const float2* __restrict__ sscal,
     constexpr int NOUT = (KIND == KIND_VEC) ? 3 : (KIND == KIND_PROB ? 7 : 1);
     float out[NOUT];
     double outd;
-    gather_voxel<KIND>(x, y, z, n, F, fma != 0, svec, sscal, sprob, sdat, out, &outd);
+    gather_voxel<KIND>(x, y, z, n, slo, sw, F, fma != 0, svec, sscal, sprob, sdat, out, &outd);
     const size_t n3 = (size_t)n * n * n;
     const size_t v = ((size_t)x * n + y) * n + z;
     if (KIND == KIND_DATA64) {
@@ -326,15 +337,19 @@ __global__ void k_lap_to_q(const double* __restrict__ qd, float* __restrict__ q,
 __global__ void k_build_src(const double* __restrict__ data, const float* __restrict__ vec,
                             const float* __restrict__ prob, size_t n3, int mode, double ave,
                             float4* __restrict__ svec, float2* __restrict__ sscal,
-                            float4* __restrict__ sprob) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3) return;
+                            float4* __restrict__ sprob, int n, int lo, int w) {
+    // one thread per voxel of the box [lo, lo + w)^3: j indexes the compact
+    // arrays, i the grid
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= (size_t)w * w * w) return;
+    const int bz = (int)(j % w), by = (int)((j / w) % w), bx = (int)(j / ((size_t)w * w));
+    const size_t i = ((size_t)(lo + bx) * n + (lo + by)) * n + (lo + bz);
     const double d = data[i];
     const float mask = (d != 0.0) ? 1.f : 0.f;
     if (mode == VX_MODE_VEC || mode == VX_MODE_PROB) {
-        svec[i] = make_float4(vec[3 * i], vec[3 * i + 1], vec[3 * i + 2], mask);
+        svec[j] = make_float4(vec[3 * i], vec[3 * i + 1], vec[3 * i + 2], mask);
         if (mode == VX_MODE_PROB)
-            sprob[i] = make_float4(prob[i], prob[n3 + i], prob[2 * n3 + i], prob[3 * n3 + i]);
+            sprob[j] = make_float4(prob[i], prob[n3 + i], prob[2 * n3 + i], prob[3 * n3 + i]);
     } else {
         // query channel value per mode (VESPER_1_prob_new.py:548-553), cast
         // to float32 as np.copyto into the float32 FFT input does
@@ -344,7 +359,7 @@ __global__ void k_build_src(const double* __restrict__ data, const float* __rest
             else if (mode == VX_MODE_CC) v = (float)d;
             else if (mode == VX_MODE_PCC) v = (float)(d - ave);
         }
-        sscal[i] = make_float2(v, mask);
+        sscal[j] = make_float2(v, mask);
     }
 }
 
@@ -1951,14 +1966,15 @@ int launch_xcorr(const float2* spec, const float2* tgt, float2* out, const FftDe
 
 int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
                   const double* sdat, const float* rot, int n, int nb, int fma, float* q,
-                  double* qd, unsigned long long* keys, int nkeys, int lo, int w, hipStream_t st) {
+                  double* qd, unsigned long long* keys, int nkeys, int lo, int w, int slo, int sw,
+                  hipStream_t st) {
     const int nzb = (w + 63) / 64;
     dim3 grid(nzb * ((w + 3) / 4), w, nb);
     switch (kind) {
-        case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
-        case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
-        case KIND_PROB:   hipLaunchKernelGGL(k_gather<KIND_PROB>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
-        default:          hipLaunchKernelGGL(k_gather<KIND_DATA64>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w); break;
+        case KIND_VEC:    hipLaunchKernelGGL(k_gather<KIND_VEC>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w, slo, sw); break;
+        case KIND_SCALAR: hipLaunchKernelGGL(k_gather<KIND_SCALAR>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w, slo, sw); break;
+        case KIND_PROB:   hipLaunchKernelGGL(k_gather<KIND_PROB>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w, slo, sw); break;
+        default:          hipLaunchKernelGGL(k_gather<KIND_DATA64>, grid, dim3(256), 0, st, svec, sscal, sprob, sdat, rot, n, nzb, fma, q, qd, keys, nkeys, lo, w, slo, sw); break;
     }
     VX_LAUNCH_CHECK();
     return VX_OK;
@@ -1994,6 +2010,7 @@ struct vx_plan {
     float* rot_stage;
     bool have_target, have_search, mix;
     bool q_dirty;     // q / qd hold non-zero values outside the support box
+    int src_lo, src_w; // box of the compact gather source (svec / sscal / sprob)
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
     int xvariant;     // x-pass kernel: 0 register FFT, 1 LDS FFT, 2 MFMA DFT, 3 register-prefetch FFT
     int xT;           // x-pass tile width (columns), 0 = default
@@ -2171,13 +2188,8 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
     const size_t n3 = (size_t)p->n * p->n * p->n;
     if (p->kind == KIND_VEC || p->kind == KIND_PROB) VX_REQUIRE(svec, VX_ERR_ARG, "vec required");
     if (p->kind == KIND_PROB) VX_REQUIRE(sprob, VX_ERR_ARG, "prob channels required");
-    if (p->kind == KIND_DATA64) {
+    if (p->kind == KIND_DATA64)
         VX_HIP(hipMemcpyAsync(p->sdat, sdata, n3 * sizeof(double), hipMemcpyDeviceToDevice, st));
-    } else {
-        hipLaunchKernelGGL(k_build_src, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st,
-                           sdata, svec, sprob, n3, p->mode, t_ave, p->svec, p->sscal, p->sprob);
-        VX_LAUNCH_CHECK();
-    }
     // Support box of every rotation of the query: a destination voxel d takes
     // source voxel floor(F (i - c) + c) with |i - d| <= 1 per axis (odd-N
     // fold), so a non-zero value lands only within radius R_s + 2 sqrt(3) of c,
@@ -2213,6 +2225,16 @@ extern "C" int vx_set_search(vx_plan* p, const double* sdata, const float* svec,
         p->lo = lo;
         p->w = hi - lo + 1;
     }
+    // the gather's source: the box of the query, compact (every non-zero voxel
+    // lies inside it)
+    if (p->kind != KIND_DATA64) {
+        const size_t nb3 = (size_t)p->w * p->w * p->w;
+        hipLaunchKernelGGL(k_build_src, dim3((unsigned)((nb3 + 255) / 256)), dim3(256), 0, st, sdata, svec, sprob,
+                           n3, p->mode, t_ave, p->svec, p->sscal, p->sprob, p->n, p->lo, p->w);
+        VX_LAUNCH_CHECK();
+    }
+    p->src_lo = p->lo;
+    p->src_w = p->w;
     p->have_search = true;
     return ensure_dft(p);
 }
@@ -2377,7 +2399,8 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
 #define VX_MARK(i) do { if (ev) VX_HIP(hipEventRecord(ev[i], st)); } while (0)
     VX_MARK(0);
     int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot + (size_t)r0 * 9, n, nb,
-                           (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, ws.q, ws.qd, ws.keys, nk, p->lo, p->w, st);
+                           (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, ws.q, ws.qd, ws.keys, nk, p->lo, p->w, p->src_lo,
+                           p->src_w, st);
     if (rc != VX_OK) return rc;
     if (p->kind == KIND_DATA64) {
         hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), nb), dim3(256), 0, st, ws.qd, ws.q, n);
