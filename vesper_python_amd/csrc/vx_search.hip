@@ -1368,7 +1368,8 @@ k_fwd_z(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int
     const int nhz = SP::kStatic ? SP::kN / 2 + 1 : nhz_;
     float2* A = lds;
     float2* Bf = lds + P * n;
-    const FftDesc fd = stage_twiddles(fdg, lds + 2 * P * n);
+    // the in-place compile-time FFT needs no ping-pong buffer: twiddles follow A
+    const FftDesc fd = stage_twiddles(fdg, lds + (SP::kStatic ? 1 : 2) * P * n);
     const int x = lo + blockIdx.y;
     const size_t plane = blockIdx.z;
     const int l0 = (lo >> 1) + blockIdx.x * P;
@@ -1459,7 +1460,8 @@ k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G, in
     __shared__ uint32_t s_i[16];
     float2* A = lds;
     float2* Bf = lds + P * n;
-    const FftDesc fd = stage_twiddles(fdg, lds + 2 * P * n);
+    // the in-place compile-time FFT needs no ping-pong buffer: twiddles follow A
+    const FftDesc fd = stage_twiddles(fdg, lds + (SP::kStatic ? 1 : 2) * P * n);
     const int x = blockIdx.y;
     const int bg = blockIdx.z;
     const int npair = (n + 1) >> 1;
@@ -1584,8 +1586,9 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
     const dim3 gz((l1 - l0 + P - 1) / P, w, nplanes);
     auto zgo = [&](auto sp) {
         using SP = decltype(sp);
-        allow_lds_any(k_fwd_z<SP>, lz);
-        hipLaunchKernelGGL(k_fwd_z<SP>, gz, dim3(256), lz, st, q, spec, fd, n, nhz, lo, w, P);
+        const size_t l = SP::kStatic ? ((size_t)P * n + fd.ntw) * sizeof(float2) : lz;
+        allow_lds_any(k_fwd_z<SP>, l);
+        hipLaunchKernelGGL(k_fwd_z<SP>, gz, dim3(256), l, st, q, spec, fd, n, nhz, lo, w, P);
     };
     if (!(use_static && n * P <= 256 * kPencilMaxV &&
           (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
@@ -1606,8 +1609,9 @@ int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, uns
     const dim3 gz((npair + P - 1) / P, n, nbg);
     auto zgo = [&](auto sp) {
         using SP = decltype(sp);
-        allow_lds_any(k_inv_z<SP>, lz);
-        hipLaunchKernelGGL(k_inv_z<SP>, gz, dim3(256), lz, st, sum, fd, n, nhz, G, P, keys, nkeys, real_out);
+        const size_t l = SP::kStatic ? ((size_t)P * n + fd.ntw) * sizeof(float2) : lz;
+        allow_lds_any(k_inv_z<SP>, l);
+        hipLaunchKernelGGL(k_inv_z<SP>, gz, dim3(256), l, st, sum, fd, n, nhz, G, P, keys, nkeys, real_out);
     };
     if (!(use_static && n * P <= 256 * kPencilMaxV &&
           (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
