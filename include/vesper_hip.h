@@ -4,8 +4,11 @@
  *
  * Conventions
  *   - Every pointer named *_dev is device memory owned by the caller; *_host
- *     is host memory.  Plans own their workspaces; hot calls
- *     (vx_score_rows, vx_topn) allocate nothing.
+ *     is host memory.  Plans own their workspaces; vx_score_rows and
+ *     vx_topn_ws allocate nothing.  vx_topn, vx_map_stats, vx_get_score and
+ *     vx_threshold_dmax share a library-owned scratch that grows on demand
+ *     (hipDeviceSynchronize + hipFree + hipMalloc when a call needs more than
+ *     every earlier one): never record them into a HIP graph.
  *   - All work is enqueued on the caller's stream (a hipStream_t passed as
  *     void*; NULL = the null stream).  Calls that return host values
  *     (vx_map_stats, vx_get_score, vx_plan_info) synchronise that stream.
@@ -173,10 +176,22 @@ int vx_score_rows_profile(vx_plan* plan, const float* rot_dev, int nrows, float*
  * Top-N rows by (score descending, row ascending) -- the stable
  * sorted(..., reverse=True)[:TopN] of VESPER_1_prob_new.py:664, :739, :992,
  * :1082.  Scores are scores_dev[row*stride + col]; rows_dev receives
- * min(topn, nrows) row indices.  topn <= 1024.
+ * min(topn, nrows) row indices.  topn <= 1024.  The key buffers come from the
+ * library's grow-on-demand scratch (see Conventions).
  */
 int vx_topn(const float* scores_dev, int nrows, int stride, int col, int topn,
             int32_t* rows_dev, vx_stream stream);
+
+/*
+ * vx_topn on a caller-owned workspace (for HIP-graph capture and callers that
+ * keep their own memory): ws_dev holds at least *bytes_host bytes as
+ * vx_topn_ws_bytes gives them for (nrows, topn), 8-byte aligned.  Allocates
+ * nothing; results equal vx_topn's.  Calls sharing one workspace must not
+ * overlap.
+ */
+int vx_topn_ws_bytes(int nrows, int topn, int64_t* bytes_host);
+int vx_topn_ws(const float* scores_dev, int nrows, int stride, int col, int topn,
+               int32_t* rows_dev, void* ws_dev, int64_t ws_bytes, vx_stream stream);
 
 /*
  * One rotation of the query map: rot_mrc (VESPER_1_prob_new.py:279-354) or,

@@ -49,3 +49,13 @@ def test_errors_are_status_codes_not_aborts():
     rc = lib.vx_topn(None, 10, 1, 0, 5, None, None)
     assert rc == -1
     assert lib.vx_plan_destroy(None) == 0
+
+
+def test_topn_workspace_size_is_host_only():
+    lib = _lib()
+    nb = ctypes.c_int64(0)
+    assert lib.vx_topn_ws_bytes(24624, 10, ctypes.byref(nb)) == 0
+    # two buffers of (13 chunks x 10 keys + 2048) 8-byte keys
+    assert nb.value == 2 * (13 * 10 + 2048) * 8
+    assert lib.vx_topn_ws_bytes(0, 10, ctypes.byref(nb)) == -1
+    assert lib.vx_topn_ws_bytes(10, 1025, ctypes.byref(nb)) == -1

@@ -50,6 +50,50 @@ def test_graph_replay_matches_eager(gpu):
         torch.cuda.synchronize()
 
 
+def test_graph_replay_survives_library_scratch_growth(gpu):
+    """A captured step's top-k buffers are its own: a larger vx_topn after the
+    capture (which grows, frees and reallocates the library's shared scratch)
+    leaves later replays bit-identical to the eager step."""
+    n = 20
+    rng = np.random.default_rng(23)
+    td, tv = _maps(rng, n)
+    sd, sv = _maps(rng, n)
+    rot = torch.from_numpy(rotation_rows(rng.uniform(0, 360, (24, 3)))).cuda()
+    plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=8)
+    plan.set_target(engine.to_dev(td, torch.float64), engine.to_dev(tv, torch.float32))
+    plan.set_search(engine.to_dev(sd, torch.float64), engine.to_dev(sv, torch.float32))
+    eb, _ = plan.score_rows(rot)
+    erows = engine.topn(eb[:, 0].contiguous(), 10)
+    gs = batch.GraphedSearch(plan, rot, 10)
+    for rep, big in enumerate((300_000, 900_000)):
+        # a larger selection than any before: the shared scratch is regrown
+        scores = torch.rand(big, device="cuda")
+        rows = engine.topn(scores, 1024)
+        assert np.array_equal(rows[:5], torch.sort(scores, descending=True).indices[:5].cpu().numpy())
+        torch.cuda.synchronize()
+        print("[case] replay after scratch growth", rep, flush=True)
+        _, _, grows = gs.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(grows.cpu().numpy(), erows)
+
+
+def test_topn_ws_equals_topn(gpu):
+    rng = np.random.default_rng(24)
+    for nrows, k, stride, col in ((5000, 10, 1, 0), (24624, 10, 3, 2), (70000, 1024, 2, 1), (7, 10, 1, 0)):
+        s = rng.standard_normal(nrows * stride).astype(np.float32)
+        s[rng.integers(0, nrows * stride, nrows // 4)] = 0.5          # ties keep row order
+        sd = torch.from_numpy(s).cuda()
+        want = engine.topn(sd, k, stride, col)
+        ws = engine.topn_workspace(nrows, k)
+        rows = torch.empty(max(k, 1), dtype=torch.int32, device="cuda")
+        engine.topn_into(sd, nrows, k, rows, ws, stride, col)
+        got = rows[:min(k, nrows)].cpu().numpy()
+        assert np.array_equal(got, want)
+        assert list(got) == ref.stable_topn(list(s[col::stride]), k)
+    with pytest.raises(nat.VesperError):
+        engine.topn_into(sd, 7, 10, rows, ws[:1], 1, 0)
+
+
 def _resampled_pair(tmp, seed, a_edge, natoms, s):
     p = synth.make_pair(seed, a_edge=a_edge, natoms=natoms, b_edge=40, nsub=80)
     paths = synth.write_pair(p, str(tmp / f"p{seed}"))

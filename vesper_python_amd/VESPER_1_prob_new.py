@@ -80,7 +80,8 @@ def mrc_set_vox_size(mrc, th=0.00, voxel_size=7.0):
     mrc._dev_data = d
     dmax = np.float64(dmax)
     print()
-    print("#dmax=" + str(dmax))
+    # the reference prints the voxel-unit extent divided by the voxel width
+    print("#dmax=" + str(dmax / mrc.xwidth))
     dmax = dmax * mrc.xwidth
     new_cent = mrc.cent * mrc.xwidth + mrc.orig
     tmp_size = 2 * dmax / voxel_size

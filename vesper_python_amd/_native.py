@@ -48,6 +48,8 @@ SIGNATURES = {
     "vx_score_rows": (_I, [_P, _P, _I, _P, _P, _P]),
     "vx_score_rows_profile": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "vx_topn": (_I, [_P, _I, _I, _I, _I, _P, _P]),
+    "vx_topn_ws_bytes": (_I, [_I, _I, _P]),
+    "vx_topn_ws": (_I, [_P, _I, _I, _I, _I, _P, _P, ctypes.c_int64, _P]),
     "vx_rotate_map": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "vx_get_score": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P]),
     "vx_laplacian": (_I, [_I, _P, _P, _P]),

@@ -40,7 +40,10 @@ class GraphedSearch:
         self.best = torch.empty((R, plan.groups), dtype=torch.float32, device=rot.device)
         self.trans = torch.empty((R, plan.groups, 3), dtype=torch.int32, device=rot.device)
         self.rows = torch.empty(max(self.k, 1), dtype=torch.int32, device=rot.device)
-        self._step()                      # warm-up outside capture: library scratch, plan state
+        # the top-k key buffers belong to this object, so a replay never
+        # touches memory that another call could free or reuse
+        self.ws = engine.topn_workspace(R, self.k, rot.device)
+        self._step()                      # warm-up outside capture: plan state
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
@@ -50,8 +53,7 @@ class GraphedSearch:
     def _step(self):
         self.plan.score_rows(self.rot, self.best, self.trans)
         G = self.plan.groups
-        nat.call("vx_topn", engine._p(self.best), int(self.rot.shape[0]), G, G - 1, max(self.k, 1),
-                 engine._p(self.rows), engine._stream())
+        engine.topn_into(self.best, int(self.rot.shape[0]), self.k, self.rows, self.ws, stride=G, col=G - 1)
 
     def replay(self):
         self.graph.replay()

@@ -1,6 +1,6 @@
 """Build libvesper_hip.so (hand-written HIP kernels for gfx950 + the C-ABI).
 
-Usage:  python -m vesper_python_amd.build      (or __graft_entry__.build())
+Usage:  python -m vesper_python_amd.build [--incremental]   (or __graft_entry__.build())
 
 The library is built in-tree (vesper_python_amd/libvesper_hip.so) with
 explicit hipcc invocations: one object per translation unit, then a link.
@@ -34,8 +34,20 @@ CFLAGS = [
 ]
 
 
-def _compile(src: str) -> str:
+def _stale(src: str, obj: str) -> bool:
+    """True unless obj is newer than src and every header it can include."""
+    if not os.path.exists(obj):
+        return True
+    deps = [os.path.join(CSRC, src)] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".hpp")]
+    deps.append(os.path.join(INCLUDE, "vesper_hip.h"))
+    t = os.path.getmtime(obj)
+    return any(os.path.getmtime(d) >= t for d in deps)
+
+
+def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
+    if incremental and not _stale(src, obj):
+        return obj
     cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
@@ -43,10 +55,12 @@ def _compile(src: str) -> str:
     return obj
 
 
-def build(verbose: bool = True) -> str:
+def build(verbose: bool = True, incremental: bool = False) -> str:
+    """Compile every source (incremental=True: only those older than their
+    object, for development) and link the library."""
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+        objs = list(ex.map(lambda f: _compile(f, incremental), SOURCES))
     tmp = LIB + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -59,5 +73,5 @@ def build(verbose: bool = True) -> str:
 
 
 if __name__ == "__main__":
-    build()
+    build(incremental="--incremental" in sys.argv)
     sys.exit(0)

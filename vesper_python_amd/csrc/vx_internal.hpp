@@ -22,4 +22,11 @@ int launch_gather(int kind, const float4* svec, const float2* sscal, const float
                   double* qd, unsigned long long* keys, int nkeys, int lo, int w, int slo, int sw,
                   hipStream_t st);
 
+// Top-N selection (vx_score.hip): keys per ping-pong buffer, argument check,
+// and the selection on caller-provided buffers of topn_cap keys each.
+size_t topn_cap(int nrows, int topn);
+int topn_check(const float* scores, int nrows, int stride, int col, int topn, const int32_t* rows);
+int topn_run(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
+             unsigned long long* bufA, unsigned long long* bufB, hipStream_t st);
+
 }  // namespace vx

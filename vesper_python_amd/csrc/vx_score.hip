@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include "vx_common.hpp"
+#include "vx_internal.hpp"
 
 namespace vx {
 
@@ -291,25 +292,31 @@ extern "C" int vx_get_score(int n, const double* td, const float* tv, double t_a
     return VX_OK;
 }
 
-extern "C" int vx_topn(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
-                       vx_stream stream) {
+namespace vx {
+
+// Keys held by one of the two ping-pong buffers of a top-N selection: the
+// first pass writes `topn` keys per chunk of kSortN rows; later passes shrink
+// that list by kSortN / topn >= 2 each.
+size_t topn_cap(int nrows, int topn) {
+    const size_t nchunk = ((size_t)nrows + kSortN - 1) / kSortN;
+    return nchunk * (size_t)topn + kSortN;
+}
+
+int topn_check(const float* scores, int nrows, int stride, int col, int topn, const int32_t* rows) {
     VX_REQUIRE(scores && rows && nrows > 0 && stride > 0 && col >= 0 && col < stride, VX_ERR_ARG, "bad arguments");
     VX_REQUIRE(topn >= 1 && topn <= 1024, VX_ERR_ARG, "topn must be in [1, 1024]");
-    hipStream_t st = (hipStream_t)stream;
+    return VX_OK;
+}
+
+// The selection itself on caller-provided key buffers (each topn_cap keys).
+int topn_run(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
+                    unsigned long long* bufA, unsigned long long* bufB, hipStream_t st) {
     const int keep = topn;
-    const int nchunk = (nrows + kSortN - 1) / kSortN;
-    const size_t cap = (size_t)nchunk * keep + kSortN;
-    void* scr = nullptr;
-    int rc = scratch_get(2 * cap * sizeof(unsigned long long) + (size_t)kRedBlocks * 8 * sizeof(double) + 512, &scr);
-    if (rc) return rc;
-    unsigned long long* bufA = (unsigned long long*)((char*)scr + (size_t)kRedBlocks * 8 * sizeof(double) + 512);
-    unsigned long long* bufB = bufA + cap;
-    int n = nrows;
-    int nb = nchunk;
+    int nb = (nrows + kSortN - 1) / kSortN;
     hipLaunchKernelGGL(k_topn_chunk, dim3(nb), dim3(1024), 0, st, scores, stride, col,
-                       (const unsigned long long*)nullptr, n, keep, bufA);
+                       (const unsigned long long*)nullptr, nrows, keep, bufA);
     VX_LAUNCH_CHECK();
-    n = nb * keep;
+    int n = nb * keep;
     while (nb > 1) {
         nb = (n + kSortN - 1) / kSortN;
         hipLaunchKernelGGL(k_topn_chunk, dim3(nb), dim3(1024), 0, st, scores, stride, col, bufA, n, keep, bufB);
@@ -321,6 +328,39 @@ extern "C" int vx_topn(const float* scores, int nrows, int stride, int col, int 
     hipLaunchKernelGGL(k_keys_to_rows, dim3((m + 255) / 256), dim3(256), 0, st, bufA, m, rows);
     VX_LAUNCH_CHECK();
     return VX_OK;
+}
+
+}  // namespace vx
+
+extern "C" int vx_topn(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
+                       vx_stream stream) {
+    int rc = topn_check(scores, nrows, stride, col, topn, rows);
+    if (rc) return rc;
+    const size_t cap = topn_cap(nrows, topn);
+    void* scr = nullptr;
+    rc = scratch_get(2 * cap * sizeof(unsigned long long) + (size_t)kRedBlocks * 8 * sizeof(double) + 512, &scr);
+    if (rc) return rc;
+    unsigned long long* bufA = (unsigned long long*)((char*)scr + (size_t)kRedBlocks * 8 * sizeof(double) + 512);
+    return topn_run(scores, nrows, stride, col, topn, rows, bufA, bufA + cap, (hipStream_t)stream);
+}
+
+extern "C" int vx_topn_ws_bytes(int nrows, int topn, int64_t* bytes_host) {
+    VX_REQUIRE(bytes_host && nrows > 0 && topn >= 1 && topn <= 1024, VX_ERR_ARG, "bad arguments");
+    *bytes_host = (int64_t)(2 * topn_cap(nrows, topn) * sizeof(unsigned long long));
+    return VX_OK;
+}
+
+extern "C" int vx_topn_ws(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
+                          void* ws, int64_t ws_bytes, vx_stream stream) {
+    int rc = topn_check(scores, nrows, stride, col, topn, rows);
+    if (rc) return rc;
+    const size_t cap = topn_cap(nrows, topn);
+    VX_REQUIRE(ws && ((uintptr_t)ws & 7) == 0, VX_ERR_ARG, "workspace must be 8-byte aligned device memory");
+    VX_REQUIRE((size_t)ws_bytes >= 2 * cap * sizeof(unsigned long long), VX_ERR_ARG,
+               "workspace of %lld bytes < the %zu bytes vx_topn_ws_bytes gives", (long long)ws_bytes,
+               2 * cap * sizeof(unsigned long long));
+    unsigned long long* bufA = (unsigned long long*)ws;
+    return topn_run(scores, nrows, stride, col, topn, rows, bufA, bufA + cap, (hipStream_t)stream);
 }
 
 extern "C" int vx_laplacian(int n, const double* in, double* out, vx_stream stream) {

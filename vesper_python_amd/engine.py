@@ -147,6 +147,23 @@ def topn(scores: torch.Tensor, k: int, stride: int = 1, col: int = 0) -> np.ndar
     return rows[:k_eff].cpu().numpy()
 
 
+def topn_workspace(nrows: int, k: int, device=None) -> torch.Tensor:
+    """A device workspace for vx_topn_ws over `nrows` rows and k <= 1024."""
+    nb = ctypes.c_int64(0)
+    nat.call("vx_topn_ws_bytes", int(nrows), int(max(k, 1)), ctypes.byref(nb))
+    dev = device if device is not None else require_gpu()
+    return torch.empty((nb.value + 7) // 8, dtype=torch.int64, device=dev)
+
+
+def topn_into(scores: torch.Tensor, nrows: int, k: int, rows: torch.Tensor, ws: torch.Tensor,
+              stride: int = 1, col: int = 0) -> None:
+    """Rows of the k best scores into the device tensor `rows` (score desc,
+    row asc), on the caller's workspace `ws` (topn_workspace): allocates
+    nothing, so it can be recorded into a HIP graph."""
+    nat.call("vx_topn_ws", _p(scores), int(nrows), int(stride), int(col), int(max(k, 1)), _p(rows), _p(ws),
+             int(ws.numel() * ws.element_size()), _stream())
+
+
 class SearchPlan:
     """A vx_plan: target spectra + query source for one (N, mode) search."""
 
