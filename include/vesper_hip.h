@@ -194,6 +194,20 @@ int vx_topn_ws(const float* scores_dev, int nrows, int stride, int col, int topn
                int32_t* rows_dev, void* ws_dev, int64_t ws_bytes, vx_stream stream);
 
 /*
+ * vx_topn that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP-graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  Its key buffers are a
+ * 2.1 MB array in the library's code object that each GPU holds from load to
+ * process end; nrows <= 262144 (2^18), else VX_ERR_ARG.  The first call on a
+ * device resolves that array's address there (hipGetSymbolAddress), so make
+ * one call before capturing it.  All calls on one GPU share the array: calls
+ * that overlap in time on one GPU (other streams, other host threads) race.
+ * Results equal vx_topn's.
+ */
+int vx_topn_noalloc(const float* scores_dev, int nrows, int stride, int col, int topn,
+                    int32_t* rows_dev, vx_stream stream);
+
+/*
  * One rotation of the query map: rot_mrc (VESPER_1_prob_new.py:279-354) or,
  * with prob_dev != NULL, rot_mrc_prob (:1211-1299).  rot_host9 = F as above.
  * Outputs are fully written (zeros where nothing lands).

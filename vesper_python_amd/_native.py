@@ -50,6 +50,7 @@ SIGNATURES = {
     "vx_topn": (_I, [_P, _I, _I, _I, _I, _P, _P]),
     "vx_topn_ws_bytes": (_I, [_I, _I, _P]),
     "vx_topn_ws": (_I, [_P, _I, _I, _I, _I, _P, _P, ctypes.c_int64, _P]),
+    "vx_topn_noalloc": (_I, [_P, _I, _I, _I, _I, _P, _P]),
     "vx_rotate_map": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "vx_get_score": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P]),
     "vx_laplacian": (_I, [_I, _P, _P, _P]),

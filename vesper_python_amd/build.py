@@ -18,7 +18,8 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libvesper_hip.so")
-SOURCES = ["vx_search.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip"]
+SOURCES = ["vx_search.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip",
+           "vx_noalloc.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VX_OFFLOAD_ARCH", "gfx950")
