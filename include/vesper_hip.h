@@ -235,6 +235,19 @@ int vx_get_score(int n, const double* tdata_dev, const float* tvec_dev, double t
 int vx_threshold_dmax(float* data_dev, int sx, int sy, int sz, float th, double* out2_host,
                       vx_stream stream);
 
+/*
+ * The float32 FFT input planes of a map for a mode: the vector components of
+ * rot_and_search_fft / rot_and_search_fft_prob (VESPER_1_prob_new.py:1434-1438,
+ * :1485-1497), the query transform of fft_search_score_trans_1d (:546-558:
+ * Overlap 1[d>0], CC d[d>0], PCC (d - ave)[d>0], Laplacian laplacian_filter(d))
+ * and the target channels (:601-615, :831-838).  Values are formed in float64,
+ * then rounded to float32.  out_dev holds C planes [C][n][n][n]: C = 3 for
+ * VX_MODE_VEC (vec_dev [n^3 x 3]), 1 for O/C/P/L (data_dev float64 [n^3]),
+ * 7 for VX_MODE_PROB (3 vector + the 4 planes of prob_dev [4][n^3]).
+ */
+int vx_mode_planes(int n, int mode, const double* data_dev, const float* vec_dev, const float* prob_dev,
+                   double ave, float* out_dev, vx_stream stream);
+
 /* laplacian_filter (VESPER_1_prob_new.py:494-526) on an [n][n][n] float64 grid. */
 int vx_laplacian(int n, const double* in_dev, double* out_dev, vx_stream stream);
 

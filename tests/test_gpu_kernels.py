@@ -402,25 +402,77 @@ def test_fft_search_best_dot_matches_oracle(gpu, n):
     assert t == et or abs(s64[t] - eb) <= RTOL * abs(eb)
 
 
+def _target(td, tv):
+    st = ref.map_stats(td)
+    return ref.MapGrid(data=td, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=tv, ave=st["ave"],
+                       std=st["std"], std_norm_ave=st["std_norm_ave"], xdim=td.shape[0])
+
+
 @pytest.mark.parametrize("mode", ["Overlap", "CC", "PCC", "Laplacian"])
 def test_fft_search_score_trans_1d_modes(gpu, mode):
+    """fft_search_score_trans_1d (VESPER_1_prob_new.py:529-565): the query
+    transform on the device, the correlation with the caller's spectrum and
+    its argmax, against the oracle's channels (pinned by closed forms)."""
     from vesper_python_amd import VESPER_1_prob_new as V
     n = 14
     rng = np.random.default_rng(850)
-    td, _ = _maps(rng, n, 0.4)
-    sd, _ = _maps(rng, n, 0.4)
-    ave = 0.37
-    tch = {"Overlap": np.where(td > 0, 1.0, 0.0), "CC": np.where(td > 0, td, 0.0),
-           "PCC": np.where(td > 0, td - ave, 0.0), "Laplacian": ref.laplacian_filter(td)}[mode]
-    spec = ref.target_spectra([tch])
-    qch = {"Overlap": np.where(sd > 0, 1.0, 0.0), "CC": np.where(sd > 0, sd, 0.0),
-           "PCC": np.where(sd > 0, sd - ave, 0.0), "Laplacian": ref.laplacian_filter(sd)}[mode]
+    td, tv = _maps(rng, n, 0.4)
+    sd, sv = _maps(rng, n, 0.4)
+    t = _target(td, tv)
+    spec = ref.target_spectra(ref.target_channels(t, mode))
+    qch = ref.query_channels(sd, sv, mode, t.ave)[0]
     _say("score_trans_1d", mode)
-    b, t = V.fft_search_score_trans_1d(spec[0], sd, None, None, None, None, mode, ave)
-    e = ref.fft_correlate(spec, [qch.astype(np.float32)], "ref64")[0]
+    b, tr = V.fft_search_score_trans_1d(spec[0], sd, None, None, None, None, mode, t.ave)
+    e = ref.fft_correlate(spec, [np.asarray(qch, np.float32)], "ref64")[0]
     eb = float(e.max())
     assert abs(b - eb) <= RTOL * abs(eb)
-    assert abs(e[t] - eb) <= RTOL * abs(eb)
+    assert abs(e[tr] - eb) <= RTOL * abs(eb)
+
+
+@pytest.mark.parametrize("mode", ["VecProduct", "Overlap", "CC", "PCC", "Laplacian"])
+def test_mode_planes_bit_exact(gpu, mode):
+    """vx_mode_planes = the oracle's query channels rounded to float32, bit
+    for bit (formed in float64 first, Laplacian in the reference's order)."""
+    n = 13
+    rng = np.random.default_rng(860)
+    sd, sv = _maps(rng, n, 0.5)
+    sd[2, 3, 4] = -0.5                                  # a negative voxel stays 0
+    ave = 0.41
+    code = nat.MODE_CODES[mode]
+    got = engine.mode_planes(code, _dev(sd, torch.float64), _dev(sv, torch.float32), None, ave).cpu().numpy()
+    want = [np.asarray(c, np.float64).astype(np.float32) for c in ref.query_channels(sd, sv, mode, ave)]
+    assert got.shape[0] == len(want)
+    for g, w in zip(got, want):
+        assert np.array_equal(g, w)
+
+
+@pytest.mark.parametrize("mode", ["VecProduct", "Overlap", "CC", "PCC", "Laplacian"])
+def test_rot_and_search_fft_dropin(gpu, mode):
+    """rot_and_search_fft (VESPER_1_prob_new.py:1418-1459) with the
+    reference's target_list of conjugated complex128 spectra: rotated maps
+    bit-exact, the score within 1e-4 of ref32 and ref64 (CC/PCC scaled by the
+    target's norms), the translation the oracle's or an FP64 near-tie."""
+    from vesper_python_amd import VESPER_1_prob_new as V
+    n = 15
+    rng = np.random.default_rng(870)
+    td, tv = _maps(rng, n, 0.4)
+    sd, sv = _maps(rng, n, 0.4)
+    t = _target(td, tv)
+    tl = ref.target_spectra(ref.target_channels(t, mode))      # built once, as the reference does
+    for ang in ([0, 0, 0], [30, 100, 60], [200, 15, 170]):
+        _say("rot_and_search_fft", mode, ang)
+        gs, gt, gv, gd = V.rot_and_search_fft(sd, sv, ang, tl, t, mode)
+        es, et, ev, ed = ref.rot_and_search_fft(sd, sv, ang, tl, t, mode, "ref32")
+        e64, et64, _, _ = ref.rot_and_search_fft(sd, sv, ang, tl, t, mode, "ref64")
+        assert np.array_equal(gv, ev) and np.array_equal(gd, ed) and gd.dtype == ed.dtype
+        assert abs(gs - es) <= RTOL * abs(es) and abs(gs - e64) <= RTOL * abs(e64)
+        if tuple(gt) != tuple(et64):
+            corr = sum(ref.fft_correlate(tl, ref.query_channels(ed, ev, mode, t.ave), "ref64"))
+            assert abs(corr[tuple(gt)] - corr[tuple(et64)]) <= RTOL * abs(corr[tuple(et64)])
+        if mode in ("CC", "PCC"):
+            assert isinstance(gs, np.float64)
+        else:
+            assert isinstance(gs, np.float32)
 
 
 def test_slab_shapes_identical_at_n175(gpu):

@@ -144,11 +144,6 @@ def test_cli_prob_runs(gpu, prob_pair):
     assert len(res) == 3 and "###Start Refining###" in buf.getvalue()
 
 
-class _Grid:
-    def __init__(self, data, vec=None):
-        self.data, self.vec, self.xdim = data, vec, data.shape[0]
-
-
 def test_rot_and_search_fft_prob_dropin(gpu):
     """The drop-in per-row prob function (VESPER_1_prob_new.py:1462-1527),
     without and with the mixing statistics, against the oracle."""
@@ -156,9 +151,10 @@ def test_rot_and_search_fft_prob_dropin(gpu):
     rng = np.random.default_rng(660)
     td, tv, tp = _maps(rng, n)
     sd, sv, sp = _maps(rng, n)
+    # the reference's target_list: conj(rfftn) of the 3 vector and 4
+    # probability channels, complex128 (VESPER_1_prob_new.py:846-860)
     spec = ref.target_spectra([tv[..., 0], tv[..., 1], tv[..., 2]] + tp)
-    tl = {"target": _Grid(td, tv), "P1": _Grid(tp[0]), "P2": _Grid(tp[1]), "P3": _Grid(tp[2]),
-          "P4": _Grid(tp[3])}
+    tl = list(spec)
     alpha, stats = 0.3, (2.0, 0.1, 4.0, 0.2)
     for ang in ([0, 0, 0], [40, 110, 70]):
         print("[case] rot_and_search_fft_prob", ang, flush=True)

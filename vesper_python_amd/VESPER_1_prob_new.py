@@ -367,26 +367,6 @@ def show_vec(origin, sampled_mrc_vec, sampled_mrc_data, score_arr, score, sample
         f.write("".join(s + "\n" for s in lines))
 
 
-def rot_and_search_fft(data, vec, angle, target_list, mrc_target, mode="VecProduct"):
-    """VESPER_1_prob_new.py:1418-1459 for one row.  ``target_list`` is the
-    reference's conjugated target spectra; the device plan rebuilds them from
-    ``mrc_target`` (the same channels, VESPER_1_prob_new.py:601-628).
-    Returns (score before 1/N^3, trans, rotated vec, rotated data)."""
-
-    class _Q:
-        pass
-
-    q = _Q()
-    q.data, q.vec = data, vec
-    s = _Search(mrc_target, q, mode, batch=1)
-    _, trans, best = s.score(np.asarray([angle]))
-    score = np.float64(best.cpu().numpy()[0])
-    if s.scale != 1.0:
-        score = score * s.scale
-    od, ov = s.rotated(angle)
-    return score, _trans_tuple(trans[0]), ov.cpu().numpy(), od.cpu().numpy().astype(np.asarray(data).dtype)
-
-
 _CORR_PLANS = {}
 
 
@@ -397,6 +377,61 @@ def _corr_plan(n):
         p = engine.SearchPlan(n, nat.VX_MODE_CC, batch=1)
         _CORR_PLANS[n] = p
     return p
+
+
+_SPECTRA = {}
+
+
+def _spectra_dev(spectra):
+    """Device complex64 copy [C, n, n, n/2+1] of a list of conjugated target
+    spectra.  The reference builds that list once per search and hands the
+    same arrays to every row (VESPER_1_prob_new.py:621-628, :644-655), so the
+    upload is kept while the caller passes the same array objects with the
+    same contents (identity plus a strided sample of every array)."""
+    arrs = [np.asarray(t) for t in spectra]
+    key = tuple((id(a), a.shape, a.dtype.str) for a in arrs)
+    sample = [a.reshape(-1)[::max(1, a.size // 4096)].copy() for a in arrs]
+    hit = _SPECTRA.get(key)
+    if hit is not None and all(np.array_equal(x, y) for x, y in zip(hit[1], sample)):
+        return hit[2]
+    dev = torch.from_numpy(np.stack([a.astype(np.complex64) for a in arrs])).to(engine.require_gpu())
+    _SPECTRA.clear()
+    _SPECTRA[key] = (arrs, sample, dev)      # holding the arrays keeps their ids unique
+    return dev
+
+
+def _best_trans(maps, n):
+    best, i = engine.sum_argmax(list(maps))
+    return best, tuple(int(v) for v in np.unravel_index(i, (n, n, n)))
+
+
+def rot_and_search_fft(data, vec, angle, target_list, mrc_target, mode="VecProduct"):
+    """VESPER_1_prob_new.py:1418-1459 for one row on the device: rotate the
+    query (vx_rotate_map), form its FFT inputs for the mode (vx_mode_planes),
+    correlate them with the caller's conjugated target spectra
+    ``target_list`` (``[:3]`` in VecProduct mode, ``[0]`` otherwise;
+    vx_correlate), and take the channel sum's first argmax (vx_sum_argmax).
+    CC and PCC scores are scaled by 1 / std^2 and 1 / std_norm_ave^2 of
+    ``mrc_target`` (:1452-1457).  Returns (score before 1/N^3, trans, rotated
+    vec, rotated data)."""
+    if mode not in _MODE_OF:
+        raise ValueError(f"unknown mode {mode!r}")
+    d = engine.to_dev(np.asarray(data, dtype=np.float64), torch.float64)
+    v = engine.to_dev(np.asarray(vec, dtype=np.float32), torch.float32)
+    n = int(d.shape[0])
+    od, ov, _ = engine.rotate_map(rotation_rows(angle)[0], d, v)
+    code = _MODE_OF[mode]
+    ave = mrc_target.ave if mrc_target.ave is not None else 0.0
+    q = engine.mode_planes(code, od, ov, None, ave)
+    spec = _spectra_dev(target_list[:3] if mode == "VecProduct" else [target_list[0]])
+    corr = _corr_plan(n).correlate(spec, q)
+    vec_score, vec_trans = _best_trans(corr, n)
+    if mode == "CC":
+        vec_score = np.float64(vec_score) * (1.0 / mrc_target.std ** 2)
+    if mode == "PCC":
+        vec_score = np.float64(vec_score) * (1.0 / mrc_target.std_norm_ave ** 2)
+    return (vec_score, vec_trans, ov.cpu().numpy(),
+            od.cpu().numpy().astype(np.asarray(data).dtype, copy=False))
 
 
 def fft_search_best_dot(target_list, query_list, a=None, b=None, c=None, fft_object=None, ifft_object=None):
@@ -413,27 +448,23 @@ def fft_search_best_dot(target_list, query_list, a=None, b=None, c=None, fft_obj
     return [host[i] for i in range(host.shape[0])]
 
 
+_MODE_1D = {"Overlap": nat.VX_MODE_OVERLAP, "CC": nat.VX_MODE_CC, "PCC": nat.VX_MODE_PCC,
+            "Laplacian": nat.VX_MODE_LAPLACIAN}
+
+
 def fft_search_score_trans_1d(target_X, search_data, a=None, b=None, fft_object=None, ifft_object=None,
                               mode="CC", ave=None):
-    """VESPER_1_prob_new.py:529-565: the mode's query transform (Overlap,
-    CC, PCC on the host as the reference does; Laplacian with vx_laplacian),
-    the correlation with ``target_X`` (vx_correlate) and its best
-    translation (vx_sum_argmax)."""
-    x2 = copy.deepcopy(np.asarray(search_data))
-    if mode == "Overlap":
-        x2 = np.where(x2 > 0, 1.0, 0.0)
-    elif mode == "CC":
-        x2 = np.where(x2 > 0, x2, 0.0)
-    elif mode == "PCC":
-        x2 = np.where(x2 > 0, x2 - ave, 0.0)
-    elif mode == "Laplacian":
-        x2 = laplacian_filter(x2)
-    n = x2.shape[0]
+    """VESPER_1_prob_new.py:529-565 on the device: the mode's query transform
+    (vx_mode_planes), the correlation with ``target_X`` (vx_correlate) and
+    its best translation (vx_sum_argmax)."""
+    if mode not in _MODE_1D:
+        raise ValueError(f"unknown mode {mode!r}")
+    d = engine.to_dev(np.asarray(search_data, dtype=np.float64), torch.float64)
+    n = int(d.shape[0])
+    q = engine.mode_planes(_MODE_1D[mode], d, None, None, 0.0 if ave is None else float(ave))
     ts = torch.from_numpy(np.asarray(target_X).astype(np.complex64)[None])
-    q = engine.to_dev(np.asarray(x2, dtype=np.float32)[None], torch.float32)
     out = _corr_plan(n).correlate(ts.to(q.device), q)
-    best, i = engine.sum_argmax([out[0]])
-    return best, tuple(int(v) for v in np.unravel_index(i, (n, n, n)))
+    return _best_trans([out[0]], n)
 
 
 def find_best_trans_list(input_list):
@@ -464,9 +495,28 @@ def search_map_fft_prob(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, 
 
 def rot_and_search_fft_prob(data, vec, dp1, dp2, dp3, dp4, angle, target_list, alpha, vstd=None, vave=None,
                             pstd=None, pave=None):
-    """VESPER_1_prob_new.py:1462-1527 for one row; ``target_list`` must be a
-    dict with the target maps under keys "target" and "P1".."P4" (the
-    device plan builds the spectra from the maps)."""
-    from .prob import score_one_prob_row
-    return score_one_prob_row(data, vec, (dp1, dp2, dp3, dp4), angle, target_list, alpha,
-                              vstd, vave, pstd, pave)
+    """VESPER_1_prob_new.py:1462-1527 for one row on the device: rotate the
+    query and its four probability maps, each with its own mask
+    (vx_rotate_map), correlate the 3 vector components with
+    ``target_list[:3]`` and the 4 probability planes with ``target_list[3:]``
+    (the caller's conjugated spectra; vx_correlate), and take each group's
+    first argmax (vx_sum_argmax); with all four statistics also the
+    alpha-mixed z-score's (vx_mixed_argmax, :1530-1545).  Returns
+    (vec_score, vec_trans, prob_score, prob_trans, mixed_score, mixed_trans),
+    the maxima unscaled; the mixed pair is None without the statistics."""
+    if len(target_list) < 7:
+        raise ValueError("target_list must hold the 3 vector and 4 probability spectra of the target")
+    d = engine.to_dev(np.asarray(data, dtype=np.float64), torch.float64)
+    v = engine.to_dev(np.asarray(vec, dtype=np.float32), torch.float32)
+    p = engine.to_dev(np.stack([np.asarray(x, dtype=np.float32) for x in (dp1, dp2, dp3, dp4)]), torch.float32)
+    n = int(d.shape[0])
+    _, ov, op = engine.rotate_map(rotation_rows(angle)[0], d, v, p)
+    q = engine.mode_planes(nat.VX_MODE_PROB, None, ov, op)
+    corr = _corr_plan(n).correlate(_spectra_dev(target_list[:7]), q)
+    vec_score, vec_trans = _best_trans(corr[:3], n)
+    prob_score, prob_trans = _best_trans(corr[3:], n)
+    mixed_score, mixed_trans = None, None
+    if vstd is not None and vave is not None and pstd is not None and pave is not None:
+        best, i = engine.mixed_argmax(list(corr[:3]), list(corr[3:]), alpha, vstd, vave, pstd, pave)
+        mixed_score, mixed_trans = best, tuple(int(x) for x in np.unravel_index(i, (n, n, n)))
+    return vec_score, vec_trans, prob_score, prob_trans, mixed_score, mixed_trans

@@ -54,6 +54,7 @@ SIGNATURES = {
     "vx_rotate_map": (_I, [_I, _P, _P, _P, _P, _P, _P, _P, _I, _P]),
     "vx_get_score": (_I, [_I, _P, _P, _D, _P, _P, _P, _P, _P]),
     "vx_laplacian": (_I, [_I, _P, _P, _P]),
+    "vx_mode_planes": (_I, [_I, _I, _P, _P, _P, _D, _P, _P]),
     "vx_interpolate": (_I, [_P, _I, _I, _I, _I, _I, _P, _P]),
     "vx_correlate": (_I, [_P, _P, _I, _P, _P, _P]),
     "vx_sum_argmax": (_I, [_P, _I, ctypes.c_int64, _P, _P, _P]),

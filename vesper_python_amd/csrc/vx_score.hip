@@ -165,15 +165,14 @@ __global__ void k_keys_to_rows(const unsigned long long* __restrict__ keys, int 
     if (t < m) rows[t] = (int32_t)(0xffffffffu - (uint32_t)(keys[t] & 0xffffffffu));
 }
 
-__global__ void k_laplacian(int n, const double* __restrict__ a, double* __restrict__ out) {
-    const size_t n3 = (size_t)n * n * n;
-    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= n3) return;
+// laplacian_filter at one voxel (VESPER_1_prob_new.py:512-525, same addition
+// order): 6-neighbour stencil where the value is > 0, neighbours outside the
+// grid count as 0, else 0.
+__device__ __forceinline__ double laplacian_at(int n, const double* __restrict__ a, size_t v) {
     const int z = (int)(v % n), y = (int)((v / n) % n), x = (int)(v / ((size_t)n * n));
     double r = 0.0;
     const double c = a[v];
     if (c > 0) {
-        // VESPER_1_prob_new.py:512-525, same addition order
         r = -6.0 * c;
         if (x + 1 < n) r += a[v + (size_t)n * n];
         if (x - 1 >= 0) r += a[v - (size_t)n * n];
@@ -182,7 +181,45 @@ __global__ void k_laplacian(int n, const double* __restrict__ a, double* __restr
         if (z + 1 < n) r += a[v + 1];
         if (z - 1 >= 0) r += a[v - 1];
     }
-    out[v] = r;
+    return r;
+}
+
+__global__ void k_laplacian(int n, const double* __restrict__ a, double* __restrict__ out) {
+    const size_t n3 = (size_t)n * n * n;
+    const size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= n3) return;
+    out[v] = laplacian_at(n, a, v);
+}
+
+// The float32 correlation planes of a map for a mode: the FFT inputs of
+// fft_search_best_dot's vector components (VESPER_1_prob_new.py:1434-1438,
+// :1485-1497), of fft_search_score_trans_1d's mode transform (:546-558) and
+// of the target channels (:601-615, :831-838).  Each value is formed in
+// float64 as the reference's NumPy code does, then rounded to float32 by the
+// copy into the float32 FFT input.
+__global__ void __launch_bounds__(256)
+k_mode_planes(int n, int mode, const double* __restrict__ data, const float* __restrict__ vec,
+              const float* __restrict__ prob, double ave, float* __restrict__ out) {
+    const size_t n3 = (size_t)n * n * n;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    if (mode == VX_MODE_VEC || mode == VX_MODE_PROB) {
+        out[i] = vec[3 * i];
+        out[n3 + i] = vec[3 * i + 1];
+        out[2 * n3 + i] = vec[3 * i + 2];
+        if (mode == VX_MODE_PROB)
+            for (int k = 0; k < 4; ++k) out[(3 + k) * n3 + i] = prob[k * n3 + i];
+        return;
+    }
+    const double d = data[i];
+    double v = 0.0;
+    if (mode == VX_MODE_LAPLACIAN) v = laplacian_at(n, data, i);
+    else if (d > 0) {
+        if (mode == VX_MODE_OVERLAP) v = 1.0;
+        else if (mode == VX_MODE_CC) v = d;
+        else if (mode == VX_MODE_PCC) v = d - ave;
+    }
+    out[i] = (float)v;
 }
 
 // Library-owned scratch for the reductions and top-N (grow-only, per process;
@@ -361,6 +398,20 @@ extern "C" int vx_topn_ws(const float* scores, int nrows, int stride, int col, i
                2 * cap * sizeof(unsigned long long));
     unsigned long long* bufA = (unsigned long long*)ws;
     return topn_run(scores, nrows, stride, col, topn, rows, bufA, bufA + cap, (hipStream_t)stream);
+}
+
+extern "C" int vx_mode_planes(int n, int mode, const double* data, const float* vec, const float* prob, double ave,
+                              float* out, vx_stream stream) {
+    VX_REQUIRE(n > 0 && out && mode >= VX_MODE_VEC && mode <= VX_MODE_PROB, VX_ERR_ARG, "bad arguments");
+    const bool needs_vec = mode == VX_MODE_VEC || mode == VX_MODE_PROB;
+    VX_REQUIRE(!needs_vec || vec, VX_ERR_ARG, "mode %d needs vec_dev", mode);
+    VX_REQUIRE(mode != VX_MODE_PROB || prob, VX_ERR_ARG, "prob mode needs prob_dev");
+    VX_REQUIRE(needs_vec || data, VX_ERR_ARG, "mode %d needs data_dev", mode);
+    const size_t n3 = (size_t)n * n * n;
+    hipLaunchKernelGGL(k_mode_planes, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, (hipStream_t)stream, n,
+                       mode, data, vec, prob, ave, out);
+    VX_LAUNCH_CHECK();
+    return VX_OK;
 }
 
 extern "C" int vx_laplacian(int n, const double* in, double* out, vx_stream stream) {

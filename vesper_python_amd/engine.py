@@ -68,6 +68,19 @@ def map_stats(data: torch.Tensor) -> dict:
     return dict(dsum=out[0], Nact=int(out[1]), ave=out[2], std=out[3], std_norm_ave=out[4])
 
 
+def mode_planes(mode: int, data=None, vec=None, prob=None, ave: float = 0.0) -> torch.Tensor:
+    """The float32 FFT input planes [C, n, n, n] of a map for a mode
+    (vx_mode_planes): the 3 vector components (VX_MODE_VEC), the mode's
+    transform of the density (O/C/P/L), or 3 vector + 4 probability planes
+    (VX_MODE_PROB)."""
+    ref = vec if vec is not None else data
+    n = int(ref.shape[0])
+    c = {nat.VX_MODE_VEC: 3, nat.VX_MODE_PROB: 7}.get(int(mode), 1)
+    out = torch.empty((c, n, n, n), dtype=torch.float32, device=ref.device)
+    nat.call("vx_mode_planes", n, int(mode), _p(data), _p(vec), _p(prob), float(ave), _p(out), _stream())
+    return out
+
+
 def laplacian(data: torch.Tensor) -> torch.Tensor:
     out = torch.empty_like(data)
     nat.call("vx_laplacian", int(data.shape[0]), _p(data), _p(out), _stream())

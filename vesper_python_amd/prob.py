@@ -52,47 +52,6 @@ class ProbSearch:
         return od, ov, op
 
 
-class _Maps:
-    """A map given as arrays (the reference passes bare NumPy arrays to the
-    per-row functions)."""
-
-    def __init__(self, data, vec=None):
-        self.data = data
-        self.vec = vec
-        self.xdim = int(np.shape(data)[0])
-
-
-def score_one_prob_row(data, vec, dps, angle, target_list, alpha, vstd=None, vave=None, pstd=None, pave=None):
-    """rot_and_search_fft_prob (VESPER_1_prob_new.py:1462-1527) for one row on
-    the device.  ``target_list`` is a dict holding the target map ("target",
-    with .data and .vec) and its four probability maps ("P1".."P4", with
-    .data): the plan builds the conjugated spectra the reference passes in.
-    Returns (vec_score, vec_trans, prob_score, prob_trans, mixed_score,
-    mixed_trans) with the raw (unscaled) correlation maxima; the mixed pair is
-    None unless all four statistics are given."""
-    keys = ("target", "P1", "P2", "P3", "P4")
-    if not isinstance(target_list, dict) or any(k not in target_list for k in keys):
-        raise TypeError("target_list must be a dict with the target map under 'target' and its probability "
-                        "maps under 'P1'..'P4'; the device plan computes their spectra")
-    tp = tuple(target_list[k] for k in keys[1:])
-    sp = tuple(_Maps(np.asarray(d)) for d in dps)
-    s = ProbSearch(tp, target_list["target"], _Maps(np.asarray(data), np.asarray(vec)), sp, batch=1)
-    mixed = all(x is not None for x in (vstd, vave, pstd, pave))
-    if mixed:
-        s.plan.set_prob_stats(alpha, vstd, vave, pstd, pave, enable=True)
-    else:
-        s.plan.set_prob_stats(alpha, 1.0, 0.0, 1.0, 0.0, enable=False)
-    best, trans = s.score(np.asarray([angle], dtype=np.float64))
-    b = best.cpu().numpy()[0]
-    t = trans.cpu().numpy()[0]
-    out = [np.float32(b[0]), tuple(int(v) for v in t[0]), np.float32(b[1]), tuple(int(v) for v in t[1]),
-           None, None]
-    if mixed:
-        out[4] = np.float32(b[2])
-        out[5] = tuple(int(v) for v in t[2])
-    return tuple(out)
-
-
 def _row(angle, b, t, rd3, mixed):
     r = {"angle": angle,
          "vec_score": np.float64(b[0]) * rd3, "vec_trans": tuple(int(v) for v in t[0]),
