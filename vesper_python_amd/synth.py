@@ -123,9 +123,11 @@ def _finish(rng, dens, origin, width):
                     threshold=float(np.percentile(dens, 90)))
 
 
-def make_pair(seed, a_edge=100, natoms=500, b_edge=60, nsub=150, width=1.0):
+def make_pair(seed, a_edge=100, natoms=500, b_edge=60, nsub=150, width=1.0, rotation=None, translation=None):
     """One (A, B) pair: A = a_edge^3 voxels with `natoms` pseudo-atoms; B =
-    b_edge^3 voxels around a moved `nsub`-atom sub-chain."""
+    b_edge^3 voxels around a moved `nsub`-atom sub-chain.  `rotation` (3x3)
+    and `translation` (Angstrom) replace the drawn ones when given (the draws
+    still happen, so every other value of the pair is the same)."""
     rng = np.random.default_rng(seed)
     box = a_edge * width
     chain = random_walk_chain(rng, natoms, box)
@@ -136,6 +138,10 @@ def make_pair(seed, a_edge=100, natoms=500, b_edge=60, nsub=150, width=1.0):
     sub = chain[start:start + nsub]
     rot = Rotation.random(random_state=rng)
     t = rng.uniform(-10.0, 10.0, 3)
+    if rotation is not None:
+        rot = Rotation.from_matrix(np.asarray(rotation, dtype=np.float64))
+    if translation is not None:
+        t = np.asarray(translation, dtype=np.float64)
     c = sub.mean(axis=0)
     moved = rot.apply(sub - c) + c + t
     b_orig = np.floor(c / width) * width - 0.5 * b_edge * width
