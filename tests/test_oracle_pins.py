@@ -173,10 +173,8 @@ def test_gather_90_degree_is_exact_permutation():
                 s = tuple(np.floor(o + 1e-9).astype(int))
                 assert nd[x, y, z] == data[s]
                 assert np.allclose(nv[x, y, z], vec[s] @ f, atol=1e-6)
-        # norm of every moved vector is preserved
-        m = nd != 0
-        assert np.allclose(np.linalg.norm(nv[m], axis=1), np.linalg.norm(vec, axis=-1).max() * 0 +
-                           np.linalg.norm(nv[m], axis=1))
+                # the moved vector keeps its source vector's norm
+                assert np.isclose(np.linalg.norm(nv[x, y, z]), np.linalg.norm(vec[s]), atol=1e-6)
     # a rotation of 90 about x: y -> z; check one voxel explicitly against the
     # convention flip(M) = R_xyz so a transposed F would fail
     nv, nd = ref.rot_mrc(data, vec, [90, 0, 0])
@@ -193,6 +191,18 @@ def test_gather_equivalent_euler_triples_identical():
     a = ref.flip_matrix_f32([90, 30, 180]).astype(np.float64)
     b = ref.flip_matrix_f32([270, 150, 0]).astype(np.float64)
     assert np.allclose(a, b, atol=1e-7)
+    # the gathered maps: same voxels, same data, vectors within float32
+    # rounding of the two (not bit-equal) matrices; and a different rotation
+    # gives a different map
+    va, da = ref.rot_mrc(data, vec, [90, 30, 180])
+    vb, db = ref.rot_mrc(data, vec, [270, 150, 0])
+    assert np.array_equal(da != 0, db != 0) and (da != 0).sum() > 50
+    agree = np.mean(da == db)
+    assert agree > 0.999, agree
+    same = (da == db) & (da != 0)
+    assert np.allclose(va[same], vb[same], atol=1e-5)
+    _, dc = ref.rot_mrc(data, vec, [90, 30, 170])
+    assert np.mean(dc == da) < agree
 
 
 def test_gather_norm_preserved_random():
@@ -242,15 +252,35 @@ def test_argmax_first_index_on_ties():
     assert ref.find_best_trans_list([a])[1] == (1, 2, 3)
 
 
-def test_mixed_score_closed_form():
+def test_mixed_score_by_voxel_loop_and_hand_value():
+    """find_best_trans_mixed (VESPER_1_prob_new.py:1530-1545): (1 - alpha)
+    weights the vector z-score and alpha the probability one.  Checked by a
+    voxel loop with a scan for the first maximum, and on a hand-computed
+    voxel where swapping the weights or the statistics changes the value."""
     rng = np.random.default_rng(8)
     v = [rng.standard_normal((4, 4, 4)).astype(np.float32) for _ in range(3)]
     p = [rng.standard_normal((4, 4, 4)).astype(np.float32) for _ in range(4)]
-    best, trans = ref.find_best_trans_mixed(v, p, 0.3, 2.0, 0.5, 4.0, -1.0, "ref64")
-    m = 0.7 * (sum(x.astype(np.float64) for x in v) - 0.5) / 2.0 + 0.3 * (sum(x.astype(np.float64) for x in p) + 1.0) / 4.0
-    assert np.isclose(best, m.max()) and trans == np.unravel_index(m.argmax(), m.shape)
-    b32, t32 = ref.find_best_trans_mixed(v, p, 0.3, 2.0, 0.5, 4.0, -1.0, "ref32")
-    assert abs(b32 - best) < 1e-5 and t32 == trans
+    alpha, vstd, vave, pstd, pave = 0.3, 2.0, 0.5, 4.0, -1.0
+    best, bi = -np.inf, None
+    for idx in itertools.product(range(4), repeat=3):
+        zv = (sum(float(x[idx]) for x in v) - vave) / vstd
+        zp = (sum(float(x[idx]) for x in p) - pave) / pstd
+        val = (1 - alpha) * zv + alpha * zp
+        if val > best:                      # strict: the first maximum in C order wins
+            best, bi = val, idx
+    got, trans = ref.find_best_trans_mixed(v, p, alpha, vstd, vave, pstd, pave, "ref64")
+    assert np.isclose(got, best, rtol=1e-12) and trans == bi
+    b32, t32 = ref.find_best_trans_mixed(v, p, alpha, vstd, vave, pstd, pave, "ref32")
+    assert abs(b32 - best) < 1e-5 and t32 == bi
+    # hand value: vector sum 5 -> z = (5 - 1) / 2 = 2; probability sum 1 ->
+    # z = (1 - 1) / 4 = 0; mixed = (1 - 0.25) * 2 + 0.25 * 0 = 1.5
+    vv = [np.zeros((2, 2, 2), np.float32) for _ in range(3)]
+    pp = [np.zeros((2, 2, 2), np.float32) for _ in range(4)]
+    vv[0][1, 0, 1], vv[2][1, 0, 1] = 2.0, 3.0
+    pp[3][1, 0, 1] = 1.0
+    for prec in ("ref64", "ref32"):
+        m, t = ref.find_best_trans_mixed(vv, pp, 0.25, 2.0, 1.0, 4.0, 1.0, prec)
+        assert t == (1, 0, 1) and m == 1.5
 
 
 # ------------------------------------------------------------------ top-N
@@ -385,3 +415,197 @@ def test_fastvec_window_edges(fastvec_ref):
         fsiv = 1.0 / 4.0
         exp = 1.0 + (np.exp(-1.5 * off * off * fsiv) if inside else 0.0)
         assert np.isclose(d[0, 0, 0], exp, rtol=1e-12)
+
+
+# ------------------------------------------------------------------ map preparation
+
+def _laplacian_loops(a):
+    """laplacian_filter as the reference's explicit loops (VESPER_1_prob_new.py:
+    509-525): only voxels > 0, neighbours outside the grid contribute nothing."""
+    nx, ny, nz = a.shape
+    out = np.zeros_like(a)
+    for x in range(nx):
+        for y in range(ny):
+            for z in range(nz):
+                if a[x][y][z] > 0:
+                    v = -6.0 * a[x][y][z]
+                    if x + 1 < nx:
+                        v += a[x + 1][y][z]
+                    if x - 1 >= 0:
+                        v += a[x - 1][y][z]
+                    if y + 1 < ny:
+                        v += a[x][y + 1][z]
+                    if y - 1 >= 0:
+                        v += a[x][y - 1][z]
+                    if z + 1 < nz:
+                        v += a[x][y][z + 1]
+                    if z - 1 >= 0:
+                        v += a[x][y][z - 1]
+                    out[x][y][z] = v
+    return out
+
+
+def test_laplacian_filter_loops_and_closed_form():
+    rng = np.random.default_rng(30)
+    a = rng.standard_normal((7, 6, 5))            # negatives, boundary voxels, positives
+    a[rng.uniform(size=a.shape) < 0.2] = 0.0
+    assert np.array_equal(ref.laplacian_filter(a), _laplacian_loops(a))
+    # one positive voxel: -6 v there; its zero neighbours stay 0 (only
+    # positive voxels are filtered); a corner voxel sees no wrapped neighbour
+    b = np.zeros((5, 5, 5))
+    b[2, 2, 2] = 1.5
+    b[0, 0, 0] = 2.0
+    b[4, 4, 4] = -3.0
+    got = ref.laplacian_filter(b)
+    exp = np.zeros_like(b)
+    exp[2, 2, 2] = -9.0
+    exp[0, 0, 0] = -12.0
+    assert np.array_equal(got, exp)
+    # two positive neighbours see each other
+    c = np.zeros((4, 4, 4))
+    c[1, 2, 3], c[2, 2, 3] = 1.0, 4.0
+    got = ref.laplacian_filter(c)
+    assert got[1, 2, 3] == -6.0 + 4.0 and got[2, 2, 3] == -24.0 + 1.0 and np.count_nonzero(got) == 2
+
+
+def test_set_vox_size_closed_form():
+    """mrc_set_vox_size (VESPER_1_prob_new.py:59-118) by hand.  Grid 20 x 30
+    x 40 at 1.5 A, origin (1, -2, 0.5), centre (10, 15, 20) voxels.  The
+    farthest voxel above the threshold sits at (8, 18, 26): offset (-2, 3, 6),
+    dmax = 7 voxels = 10.5 A; at 2 A per voxel int(2 * 10.5 / 2) = 10 -> N =
+    10; new centre = centre * 1.5 + origin = (16, 20.5, 30.5); new origin =
+    new centre - 0.5 * 10 * 2 = (6, 10.5, 20.5)."""
+    d = np.zeros((20, 30, 40), np.float32)
+    d[8, 18, 26] = 0.9
+    d[10, 15, 20] = 0.5
+    d[19, 29, 39] = 0.2             # farther, but below the threshold
+    d[0, 0, 0] = -1.0               # negative: below the threshold
+    m = ref.MapGrid(data=d, xwidth=np.float32(1.5), orig=np.array([1.0, -2.0, 0.5], np.float32))
+    src, dst = ref.set_vox_size(m, 0.3, 2.0)
+    assert dst.xdim == 10
+    assert np.array_equal(dst.cent, [16.0, 20.5, 30.5])
+    assert np.array_equal(dst.orig, [6.0, 10.5, 20.5])
+    assert np.count_nonzero(src.data) == 2 and src.data[19, 29, 39] == 0 and src.data[0, 0, 0] == 0
+    # a negative threshold leaves the data alone and thresholds at 0 (:73-79):
+    # the 0.2 voxel now counts, offset (9, 14, 19) -> dmax = sqrt(638) voxels
+    src, dst = ref.set_vox_size(m, -0.1, 2.0)
+    assert src.data[19, 29, 39] == np.float32(0.2) and src.data[0, 0, 0] == 0
+    assert dst.xdim == ref.next_fast_len(int(2 * np.sqrt(638.0) * 1.5 / 2.0))
+    # int() truncates: dmax 7 at 1.5 A and 1.9 A per voxel gives 11.05 -> 11
+    m2 = ref.MapGrid(data=d, xwidth=np.float32(1.5), orig=np.zeros(3, np.float32))
+    assert ref.set_vox_size(m2, 0.3, 1.9)[1].xdim == 11
+    assert ref.set_vox_size(m2, 0.3, 1.3)[1].xdim == 16        # int(16.15) = 16
+
+
+def test_unify_dims_closed_form():
+    """main.py:166-178: the smaller grid takes the larger N and its origin
+    becomes centre - 0.5 * s * N."""
+    s = 2.0
+    a = ref.MapGrid(data=np.zeros((2, 2, 2)), xwidth=s, orig=np.array([0.0, 0.0, 0.0]),
+                    cent=np.array([30.0, 40.0, 50.0]), xdim=16)
+    b = ref.MapGrid(data=np.zeros((2, 2, 2)), xwidth=s, orig=np.array([9.0, 9.0, 9.0]),
+                    cent=np.array([1.0, 2.0, 3.0]), xdim=21)
+    assert ref.unify_dims(a, b, s) == 21 and a.xdim == 21 and b.xdim == 21
+    assert np.array_equal(a.orig, [30.0 - 21.0, 40.0 - 21.0, 50.0 - 21.0])
+    assert np.array_equal(b.orig, [9.0, 9.0, 9.0])
+
+
+def test_map_stats_norms_not_std():
+    """VESPER_1_prob_new.py:157-161: `std` is the L2 norm of the positive
+    voxels (not a standard deviation), std_norm_ave the L2 norm of their
+    deviations from their mean; the sum and count include negatives."""
+    d = np.array([[[3.0, 4.0], [0.0, -1.0]]])
+    st = ref.map_stats(d)
+    assert st["dsum"] == 6.0 and st["Nact"] == 3
+    assert st["ave"] == 3.5
+    assert st["std"] == 5.0                         # sqrt(9 + 16); a std would be 0.5
+    assert np.isclose(st["std_norm_ave"], np.sqrt(0.5), rtol=1e-15)
+
+
+def test_mode_channels_closed_form():
+    """O/C/P/L channels (VESPER_1_prob_new.py:546-558 query, :608-615
+    target).  PCC subtracts the given average -- the target's, also for the
+    query (:1450)."""
+    d = np.array([-1.0, 0.0, 0.5, 2.0]).reshape(1, 1, 4)
+    vec = np.arange(12, dtype=np.float32).reshape(1, 1, 4, 3)
+    t = ref.MapGrid(data=d, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=vec, ave=0.25)
+    exp = {"Overlap": [0.0, 0.0, 1.0, 1.0], "CC": [0.0, 0.0, 0.5, 2.0], "PCC": [0.0, 0.0, 0.25, 1.75],
+           "Laplacian": [0.0, 0.0, -3.0 + 2.0, -12.0 + 0.5]}
+    for mode, want in exp.items():
+        (tc,) = ref.target_channels(t, mode)
+        (qc,) = ref.query_channels(d, vec, mode, 0.25)
+        assert np.array_equal(tc.ravel(), want) and np.array_equal(qc.ravel(), want), mode
+    (qp,) = ref.query_channels(d, vec, "PCC", 1.0)        # the average handed in is the one used
+    assert np.array_equal(qp.ravel(), [0.0, 0.0, -0.5, 1.0])
+    tv = ref.target_channels(t, "VecProduct")
+    assert all(np.array_equal(tv[k], vec[..., k]) for k in range(3))
+
+
+def _fold_expected(p, n):
+    """Odd-N identity gather of one channel with its own mask: plane h takes
+    row h + 1 if that voxel is non-zero, else row h (the later row wins only
+    when it passes its filter); planes above h take d + 1; plane N-1 stays 0."""
+    h = (n - 1) // 2
+    exp = np.zeros_like(p)
+
+    def cands(d):
+        if d < h:
+            return [d]
+        if d == h:
+            return [h + 1, h]
+        if d < n - 1:
+            return [d + 1]
+        return []
+    for x, y, z in itertools.product(range(n), repeat=3):
+        for cz in cands(z):
+            hit = False
+            for cx in cands(x):
+                for cy in cands(y):
+                    if p[cx, cy, cz] != 0:
+                        exp[x, y, z] = p[cx, cy, cz]
+                        hit = True
+                        break
+                if hit:
+                    break
+            if hit:
+                break
+    return exp
+
+
+def test_rot_mrc_prob_own_masks():
+    """rot_mrc_prob (VESPER_1_prob_new.py:1245-1297) gathers each
+    probability channel with its own non-zero mask, not the density's: a
+    probability voxel where the density is 0 still moves, and at the odd-N
+    centre plane each channel picks its own surviving row."""
+    rng = np.random.default_rng(31)
+    for n in (8, 9):
+        data, vec = _rand_maps(rng, n, density=0.4)
+        probs = [(rng.uniform(0.1, 1, (n, n, n)) * (rng.uniform(size=(n, n, n)) < 0.5)).astype(np.float32)
+                 for _ in range(4)]
+        # probability where the density is zero
+        assert any(np.any((p != 0) & (data == 0)) for p in probs)
+        out = ref.rot_mrc_prob(data, vec, *probs, [0, 0, 0])
+        nv, nd = ref.rot_mrc(data, vec, [0, 0, 0])
+        assert np.array_equal(out[0], nv) and np.array_equal(out[1], nd)
+        for k in range(4):
+            exp = probs[k] if n % 2 == 0 else _fold_expected(probs[k], n)
+            assert np.array_equal(out[2 + k], exp), (n, k)
+    # 90 degrees about x on an even grid: each channel is the same
+    # permutation of itself, its zeros and the density's notwithstanding
+    n = 8
+    data, vec = _rand_maps(rng, n, density=0.3)
+    probs = [(rng.uniform(0.1, 1, (n, n, n)) * (rng.uniform(size=(n, n, n)) < 0.5)).astype(np.float32)
+             for _ in range(4)]
+    out = ref.rot_mrc_prob(data, vec, *probs, [90, 0, 0])
+    f = ref.flip_matrix_f32([90, 0, 0]).astype(np.float64)
+    moved = 0
+    for x, y, z in itertools.product(range(n), repeat=3):
+        o = f @ (np.array([x, y, z]) - n / 2) + n / 2
+        if np.all(o >= 0) and np.all(o < n):
+            sidx = tuple(np.floor(o + 1e-9).astype(int))
+            for k in range(4):
+                assert out[2 + k][x, y, z] == probs[k][sidx]
+                moved += int(probs[k][sidx] != 0 and data[sidx] == 0)
+        else:
+            assert all(out[2 + k][x, y, z] == 0 for k in range(4))
+    assert moved > 10
