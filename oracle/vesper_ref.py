@@ -107,8 +107,8 @@ def set_vox_size(m: MapGrid, th: float, voxel_size: float):
     data[data < th] = 0.0                                   # :79
     idx = np.array(np.nonzero(data)).T                      # :82
     d2 = np.linalg.norm(idx - np.asarray(m.cent), axis=1)   # :84
-    dmax = float(max(d2))                                   # :85
-    dmax = dmax * np.float64(np.float32(m.xwidth))          # :89
+    dmax_vox = max(d2)                                      # :85
+    dmax = float(dmax_vox) * np.float64(np.float32(m.xwidth))  # :89
     new_cent = (np.asarray(m.cent, np.float64) * np.float64(np.float32(m.xwidth))
                 + np.asarray(m.orig, np.float32).astype(np.float64))  # :92
     tmp_size = 2 * dmax / voxel_size                        # :94
@@ -117,6 +117,9 @@ def set_vox_size(m: MapGrid, th: float, voxel_size: float):
     src = MapGrid(data=data, xwidth=m.xwidth, orig=m.orig, cent=m.cent)
     dst = MapGrid(data=np.zeros((n, n, n)), xwidth=voxel_size, orig=new_orig,
                   cent=new_cent, xdim=n)
+    # the printed "#dmax=" value: the voxel-unit extent over the width (:88)
+    dst.extra["dmax_vox"] = dmax_vox
+    dst.extra["dmax_printed"] = str(dmax_vox / np.float32(m.xwidth))
     return src, dst
 
 

@@ -483,6 +483,7 @@ def test_set_vox_size_closed_form():
     m = ref.MapGrid(data=d, xwidth=np.float32(1.5), orig=np.array([1.0, -2.0, 0.5], np.float32))
     src, dst = ref.set_vox_size(m, 0.3, 2.0)
     assert dst.xdim == 10
+    assert dst.extra["dmax_vox"] == 7.0 and dst.extra["dmax_printed"] == str(7.0 / 1.5)
     assert np.array_equal(dst.cent, [16.0, 20.5, 30.5])
     assert np.array_equal(dst.orig, [6.0, 10.5, 20.5])
     assert np.count_nonzero(src.data) == 2 and src.data[19, 29, 39] == 0 and src.data[0, 0, 0] == 0

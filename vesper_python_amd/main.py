@@ -53,6 +53,51 @@ def alpha_is_zero(objA, objB, threshold1, threshold2, bandwidth, voxel_spacing, 
                           showPDB=showPDB, folder=folder)
 
 
+def resample_prob(objA, objB, npA, npB, threshold1, threshold2, voxel_spacing, bandwidth):
+    """The prob command's map preparation (main.py:302-361): density maps and
+    the four probability channels of each (.npy in MRC order (nz, ny, nx, 4),
+    swapped like the density), each thresholded (channels at 0) and sized on
+    its own dmax, all ten grids padded to the largest edge, then resampled
+    and smoothed by fastVEC.  Returns (mrc_N1, mrc_N2, N1p, N2p)."""
+    prob_maps = np.load(npA).astype(np.float32)
+    prob_maps_chain = np.load(npB).astype(np.float32)
+    mrc1, mrc2 = mrc_obj(objA), mrc_obj(objB)
+    mrc1_p = [mrc_obj(objA) for _ in range(4)]
+    mrc2_p = [mrc_obj(objB) for _ in range(4)]
+    for k in range(4):
+        mrc2_p[k].data = np.swapaxes(prob_maps_chain[:, :, :, k], 0, 2)
+        mrc1_p[k].data = np.swapaxes(prob_maps[:, :, :, k], 0, 2)
+    print("\n###Generating Params for Resampling Density Map 1###")
+    mrc1, mrc_N1 = mrc_set_vox_size(mrc1, threshold1, voxel_spacing)
+    print("\n###Generating Params for Resampling Density Map 2###")
+    mrc2, mrc_N2 = mrc_set_vox_size(mrc2, threshold2, voxel_spacing)
+    print("\n###Generating Params for Resampling Probability Map 1###")
+    res1 = [mrc_set_vox_size(m, th=0.0, voxel_size=voxel_spacing) for m in mrc1_p]
+    print("\n###Generating Params for Resampling Probability Map 2###")
+    res2 = [mrc_set_vox_size(m, th=0.0, voxel_size=voxel_spacing) for m in mrc2_p]
+    mrc1_p = [r[0] for r in res1]
+    mrc2_p = [r[0] for r in res2]
+    N1p = [r[1] for r in res1]
+    N2p = [r[1] for r in res2]
+    mrc_list = [mrc_N1, mrc_N2, *N1p, *N2p]
+    max_dim = np.max([m.xdim for m in mrc_list])
+    for m in mrc_list:
+        if m.xdim != max_dim:
+            m.xdim = m.ydim = m.zdim = max_dim
+            m.orig = m.cent - 0.5 * voxel_spacing * m.xdim
+    for m in mrc_list:
+        m.dens = np.zeros((max_dim ** 3, 1), dtype="float32")
+        m.data = np.zeros((max_dim, max_dim, max_dim), dtype="float32")
+        m.vec = np.zeros((max_dim, max_dim, max_dim, 3), dtype="float32")
+    print("\n###Processing MAP1 Resampling###")
+    mrc_N1 = fastVEC(mrc1, mrc_N1, dreso=bandwidth)
+    N1p = [fastVEC(s, d, dreso=bandwidth) for s, d in zip(mrc1_p, N1p)]
+    print("\n###Processing MAP2 Resampling###")
+    mrc_N2 = fastVEC(mrc2, mrc_N2, dreso=bandwidth)
+    N2p = [fastVEC(s, d, dreso=bandwidth) for s, d in zip(mrc2_p, N2p)]
+    return mrc_N1, mrc_N2, N1p, N2p
+
+
 def build_parser():
     parser = argparse.ArgumentParser()
     sub = parser.add_subparsers(dest="command")
@@ -142,42 +187,8 @@ def main(argv=None):
         if alpha == 0.0:
             return alpha_is_zero(objA, objB, threshold1, threshold2, bandwidth, voxel_spacing, angle_spacing, topN,
                                  showPDB, modeVal, evalMode, folder)
-        prob_maps = np.load(npA).astype(np.float32)
-        prob_maps_chain = np.load(npB).astype(np.float32)
-        mrc1, mrc2 = mrc_obj(objA), mrc_obj(objB)
-        mrc1_p = [mrc_obj(objA) for _ in range(4)]
-        mrc2_p = [mrc_obj(objB) for _ in range(4)]
-        for k in range(4):
-            mrc2_p[k].data = np.swapaxes(prob_maps_chain[:, :, :, k], 0, 2)
-            mrc1_p[k].data = np.swapaxes(prob_maps[:, :, :, k], 0, 2)
-        print("\n###Generating Params for Resampling Density Map 1###")
-        mrc1, mrc_N1 = mrc_set_vox_size(mrc1, threshold1, voxel_spacing)
-        print("\n###Generating Params for Resampling Density Map 2###")
-        mrc2, mrc_N2 = mrc_set_vox_size(mrc2, threshold2, voxel_spacing)
-        print("\n###Generating Params for Resampling Probability Map 1###")
-        res1 = [mrc_set_vox_size(m, th=0.0, voxel_size=voxel_spacing) for m in mrc1_p]
-        print("\n###Generating Params for Resampling Probability Map 2###")
-        res2 = [mrc_set_vox_size(m, th=0.0, voxel_size=voxel_spacing) for m in mrc2_p]
-        mrc1_p = [r[0] for r in res1]
-        mrc2_p = [r[0] for r in res2]
-        N1p = [r[1] for r in res1]
-        N2p = [r[1] for r in res2]
-        mrc_list = [mrc_N1, mrc_N2, *N1p, *N2p]
-        max_dim = np.max([m.xdim for m in mrc_list])
-        for m in mrc_list:
-            if m.xdim != max_dim:
-                m.xdim = m.ydim = m.zdim = max_dim
-                m.orig = m.cent - 0.5 * voxel_spacing * m.xdim
-        for m in mrc_list:
-            m.dens = np.zeros((max_dim ** 3, 1), dtype="float32")
-            m.data = np.zeros((max_dim, max_dim, max_dim), dtype="float32")
-            m.vec = np.zeros((max_dim, max_dim, max_dim, 3), dtype="float32")
-        print("\n###Processing MAP1 Resampling###")
-        mrc_N1 = fastVEC(mrc1, mrc_N1, dreso=bandwidth)
-        N1p = [fastVEC(s, d, dreso=bandwidth) for s, d in zip(mrc1_p, N1p)]
-        print("\n###Processing MAP2 Resampling###")
-        mrc_N2 = fastVEC(mrc2, mrc_N2, dreso=bandwidth)
-        N2p = [fastVEC(s, d, dreso=bandwidth) for s, d in zip(mrc2_p, N2p)]
+        mrc_N1, mrc_N2, N1p, N2p = resample_prob(objA, objB, npA, npB, threshold1, threshold2, voxel_spacing,
+                                                 bandwidth)
         return search_map_fft_prob(*N1p, mrc_N1, mrc_N2, *N2p, ang=angle_spacing, alpha=alpha, TopN=topN,
                                    num_proc=num_processors, vave=vave, vstd=vstd, pave=pave, pstd=pstd,
                                    showPDB=showPDB, folder=folder)

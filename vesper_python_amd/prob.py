@@ -90,8 +90,11 @@ def run_prob_search(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, mrc_
     bh = best.cpu().numpy()
     th = trans.cpu().numpy()
     sorted_topN = [_row(angle_comb[i], bh[i], th[i], s.rd3, True) for i in top]
-    for idx, x in enumerate(sorted_topN):
-        print("M", str(idx + 1), x)
+    # the reference prints the row dicts under numpy 1.21 (requirements.txt),
+    # whose scalar reprs carry no type name
+    with np.printoptions(legacy="1.25"):
+        for idx, x in enumerate(sorted_topN):
+            print("M", str(idx + 1), x)
 
     if ang > 5.0:
         print("\n###Start Refining###")
