@@ -163,6 +163,33 @@ int vx_score_rows(vx_plan* plan, const float* rot_dev, int nrows, float* best_de
                   int32_t* trans_dev, vx_stream stream);
 
 /*
+ * FP64 re-scoring of rows (north-star precision path, no counterpart in the
+ * reference, which ranks by its float32 maxima, VESPER_1_prob_new.py:664,
+ * :739): for every row of rot_dev [nrows][9], each translation whose float32
+ * correlation (the search's own, per output group; prob mode's mixed score as
+ * the float32 search forms it) is >= best - window * |best| is a candidate,
+ * and its correlation N^3 sum_v T[v] Q[(v + t) mod N] is summed directly in
+ * FP64: target channels in float64 from tdata/tvec/tprob (the mode's
+ * transform, t_ave for PCC), the rotated query with float32 positions and
+ * FP64-rotated vectors from sdata/svec/sprob -- the same maps as
+ * vx_set_target / vx_set_search.  Writes, per row and group (layout of
+ * vx_score_rows; prob mode's mixed score with the vx_set_prob_stats
+ * statistics in FP64):
+ *   score_dev[row*G + g]           FP64 value of the best candidate (largest,
+ *                                  then smallest C-order index)
+ *   trans_dev[(row*G + g)*3 + a]   its translation
+ *   ncand_dev[row*G + g]           candidates found; > max_cand means only
+ *                                  max_cand of them were summed and the row
+ *                                  needs a smaller window or a larger max_cand
+ * The first call, and any larger one, grows a plan-owned workspace (batch x
+ * groups x n^3 float32 maps plus candidate lists).  Not for the MFMA x pass.
+ */
+int vx_rescore_f64(vx_plan* plan, const double* tdata_dev, const float* tvec_dev, const float* tprob_dev,
+                   double t_ave, const double* sdata_dev, const float* svec_dev, const float* sprob_dev,
+                   const float* rot_dev, int nrows, double window, int max_cand, double* score_dev,
+                   int32_t* trans_dev, int32_t* ncand_dev, vx_stream stream);
+
+/*
  * vx_score_rows with a HIP event recorded on `stream` between the launches of
  * every batch; the stream is synchronised after each batch.  stage_ms_host[6]
  * receives the summed milliseconds of {gather, forward z/y FFT, x pass with

@@ -515,12 +515,25 @@ def stable_topn(scores, topn):
     return order[:topn]
 
 
+def recheck_rows(scores, top, rel=1e-4):
+    """Rows whose rank at the top-N cutoff is re-decided in FP64: every row
+    scoring >= s_N - rel * max(|s_N|, |s_1|), s_1 and s_N the first and last
+    scores of the top N, in row order.  (The FP64 re-check of the north star;
+    the reference itself ranks by its float32 scores.)"""
+    s1, sn = scores[top[0]], scores[top[-1]]
+    thr = sn - rel * max(abs(sn), abs(s1))
+    return [i for i in range(len(scores)) if scores[i] >= thr]
+
+
 def search_map_fft(target: MapGrid, search: MapGrid, TopN=10, ang=30, mode="VecProduct",
-                   prec="ref32", fma=True, rows=None, refine=True):
+                   prec="ref32", fma=True, rows=None, refine=True, fp64_recheck=False):
     """``search_map_fft`` (VESPER_1_prob_new.py:568-795) without printing.
 
     Returns dict(angles, scores (1/N^3-scaled float64), trans, topn (row ids),
-    refined: list of dicts with angle/vec_score/vec_trans/vec/data)."""
+    refined: list of dicts with angle/vec_score/vec_trans/vec/data).
+    fp64_recheck: the rows near the top-N cutoff (recheck_rows) are re-scored
+    with prec="ref64" and the top N taken by those scores (ties: row order);
+    every refinement row is scored with "ref64"."""
     rd3 = 1.0 / (target.xdim ** 3)
     tspecs = target_spectra(target_channels(target, mode))
     angles = calc_angle_comb(ang) if rows is None else np.asarray(rows, np.float64)
@@ -533,20 +546,38 @@ def search_map_fft(target: MapGrid, search: MapGrid, TopN=10, ang=30, mode="VecP
     top = stable_topn(list(scores), TopN)
     out = dict(angles=angles, scores=scores, trans=trans, topn=top, refined=None,
                ave=np.mean(scores), std=np.std(scores))
+    top_score = {i: scores[i] for i in top}
+    top_trans = {i: tuple(trans[i]) for i in top}
+    if fp64_recheck:
+        near = recheck_rows(scores, top)
+        s64 = {}
+        for i in near:
+            s, t, _, _ = rot_and_search_fft(search.data, search.vec, angles[i], tspecs, target, mode, "ref64", fma)
+            s64[i] = (s * rd3, tuple(t))
+        top = sorted(near, key=lambda i: (-s64[i][0], i))[:TopN]
+        top_score = {i: s64[i][0] for i in top}
+        top_trans = {i: s64[i][1] for i in top}
+        out.update(topn=top, recheck=near, recheck_scores=s64)
+    out["topn_scores"] = [top_score[i] for i in top]
+    out["topn_trans"] = [top_trans[i] for i in top]
     if not refine:
         return out
     if ang > 5.0:
         ref_rows = np.concatenate([refine_angles(angles[i]) for i in top], axis=0)
         refined = []
+        rprec = "ref64" if fp64_recheck else prec
         for a in ref_rows:
-            s, t, nv, nd = rot_and_search_fft(search.data, search.vec, a, tspecs, target, mode, prec, fma)
+            s, t, nv, nd = rot_and_search_fft(search.data, search.vec, a, tspecs, target, mode, rprec, fma)
+            if fp64_recheck:
+                # the returned maps are the reference's float32 rotation
+                nv, nd = rot_mrc(search.data, search.vec, a, fma, "ref32")
             refined.append(dict(angle=tuple(a), vec_score=s * rd3, vec_trans=t, vec=nv, data=nd))
         order = stable_topn([r["vec_score"] for r in refined], TopN)
         out["refine_rows"] = ref_rows
         out["refined"] = [refined[i] for i in order]
     else:
-        out["refined"] = [dict(angle=tuple(angles[i]), vec_score=scores[i],
-                               vec_trans=tuple(trans[i])) for i in top]
+        out["refined"] = [dict(angle=tuple(angles[i]), vec_score=top_score[i],
+                               vec_trans=top_trans[i]) for i in top]
     return out
 
 

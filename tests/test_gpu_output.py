@@ -87,7 +87,9 @@ def _reference_text(argv, paths, pair, s, g, ang, topn):
             sum=st["dsum"], cnt=st["Nact"], ave=st["ave"], std=st["std"], std_norm=st["std_norm_ave"]))
     out.append("")
     t, q = grids[0][1], grids[1][1]
-    res = ref.search_map_fft(t, q, TopN=topn, ang=ang, mode="VecProduct", prec="ref32")
+    # the product's default ranking: rows near the top-N cutoff and every
+    # refinement row decided in FP64
+    res = ref.search_map_fft(t, q, TopN=topn, ang=ang, mode="VecProduct", prec="ref32", fp64_recheck=True)
     out += ["###Start Searching###", "", "Std= " + str(res["std"]) + " Ave= " + str(res["ave"]), ""]
 
     def model(prefix, i, angle, trans):
@@ -96,7 +98,7 @@ def _reference_text(argv, paths, pair, s, g, ang, topn):
                 + ") Translation= (" + "{:.3f}".format(nt[0]) + " " + "{:.3f}".format(nt[1]) + " "
                 + "{:.3f}".format(nt[2]) + ")")
     for i, r in enumerate(res["topn"]):
-        out.append(model("M", i, res["angles"][r], tuple(res["trans"][r])))
+        out.append(model("M", i, res["angles"][r], res["topn_trans"][i]))
     out += ["", ""]
     for i, r in enumerate(res["refined"]):
         out += ["", model("#", i, r["angle"], r["vec_trans"])]

@@ -60,7 +60,9 @@ def test_search_top10_matches_oracle(gpu, pair_files, mode):
     buf = io.StringIO()
     with contextlib.redirect_stdout(buf):
         got = V.search_map_fft(n1, n2, TopN=10, ang=30.0, mode=mode)
-    exp = ref.search_map_fft(_grid(n1), _grid(n2), TopN=10, ang=30.0, mode=mode, prec="ref32")
+    # the product's default ranking re-decides the rows near the cutoff and
+    # the refinement rows in FP64; the oracle does the same
+    exp = ref.search_map_fft(_grid(n1), _grid(n2), TopN=10, ang=30.0, mode=mode, prec="ref32", fp64_recheck=True)
     # refined top-10: rotations identical, translations identical, scores 1e-4
     assert [tuple(float(v) for v in r["angle"]) for r in got] == \
            [tuple(float(v) for v in r["angle"]) for r in exp["refined"]]

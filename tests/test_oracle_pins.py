@@ -610,3 +610,23 @@ def test_rot_mrc_prob_own_masks():
         else:
             assert all(out[2 + k][x, y, z] == 0 for k in range(4))
     assert moved > 10
+
+
+def test_fp64_recheck_equals_fp64_everywhere():
+    """The FP64 re-check scores only the rows near the float32 top-N cutoff
+    in FP64; its top N must equal the top N of scoring every row in FP64."""
+    n = 10
+    rng = np.random.default_rng(40)
+    td, tv = _rand_maps(rng, n, density=0.4)
+    sd, sv = _rand_maps(rng, n, density=0.4)
+    st = ref.map_stats(td)
+    t = ref.MapGrid(data=td, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=tv, ave=st["ave"],
+                    std=st["std"], std_norm_ave=st["std_norm_ave"], xdim=n)
+    s = ref.MapGrid(data=sd, xwidth=np.float32(1), orig=np.zeros(3, np.float32), vec=sv, xdim=n)
+    rows = ref.calc_angle_comb(30)[::3]
+    got = ref.search_map_fft(t, s, TopN=10, mode="VecProduct", rows=rows, refine=False, fp64_recheck=True)
+    full = ref.search_map_fft(t, s, TopN=10, mode="VecProduct", rows=rows, refine=False, prec="ref64")
+    assert got["topn"] == full["topn"]
+    assert np.allclose(got["topn_scores"], [full["scores"][i] for i in full["topn"]], rtol=1e-12)
+    assert got["topn_trans"] == [tuple(full["trans"][i]) for i in full["topn"]]
+    assert set(full["topn"]) <= set(got["recheck"]) and len(got["recheck"]) < len(rows)

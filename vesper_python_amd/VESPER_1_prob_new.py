@@ -34,6 +34,15 @@ __all__ = ["mrc_obj", "mrc_set_vox_size", "fastVEC", "doFastVEC", "rot_mrc", "ro
 # grid edge; overridable by callers)
 SEARCH_BATCH = None
 
+# FP64 re-check of the ranking: the rows near the top-N cutoff (within
+# FP64_RECHECK_REL of the N-th float32 score, relative to the larger of the
+# first and N-th) and every refinement row are re-scored in FP64 on the
+# device (vx_rescore_f64), and the top N are taken by those scores.  False:
+# rank by the float32 scores alone, as the reference does
+# (VESPER_1_prob_new.py:664, :739).
+FP64_RECHECK = True
+FP64_RECHECK_REL = 1e-4
+
 
 class mrc_obj:
     """Density data and statistics of an MRC file (VESPER_1_prob_new.py:18-56)."""
@@ -221,9 +230,35 @@ class _Search:
             b = b * self.scale
         return b * self.rd3, trans[:, 0, :].cpu().numpy(), best[:, 0].contiguous()
 
+    def rescore(self, angles):
+        """FP64 (scores 1/N^3-scaled, trans [R, 3]) of the rows (vx_rescore_f64)."""
+        rot = torch.from_numpy(rotation_rows(angles)).to(self.tdata.device)
+        ave = self.target.ave if self.target.ave is not None else 0.0
+        sc, tr, _ = self.plan.rescore_f64(rot, (self.tdata, self.tvec), (self.sdata, self.svec), ave)
+        b = sc[:, 0].cpu().numpy()
+        if self.scale != 1.0:
+            b = b * self.scale
+        return b * self.rd3, tr[:, 0, :].cpu().numpy()
+
     def rotated(self, angle):
         od, ov, _ = engine.rotate_map(rotation_rows(angle)[0], self.sdata, self.svec)
         return od, ov
+
+
+def recheck_rows(scores, top, rel=None):
+    """Rows within `rel` (default FP64_RECHECK_REL) of the N-th score of the
+    float32 top N (relative to the larger of the first and N-th), in row
+    order: the rows whose rank the FP64 re-check re-decides."""
+    rel = FP64_RECHECK_REL if rel is None else rel
+    s1, sn = scores[top[0]], scores[top[-1]]
+    thr = sn - rel * max(abs(sn), abs(s1))
+    return np.flatnonzero(np.asarray(scores) >= thr)
+
+
+def _fp64_top(rows, s64, topn):
+    """Top N of `rows` by FP64 score, ties in row order."""
+    order = sorted(range(len(rows)), key=lambda k: (-s64[k], int(rows[k])))
+    return order[:topn]
 
 
 def _trans_tuple(t):
@@ -262,8 +297,21 @@ def search_map_fft(mrc_target, mrc_search, TopN=10, ang=30, mode="VecProduct", i
     std = np.std(scores)
     print("\nStd= " + str(std) + " Ave= " + str(ave) + "\n")
     top = engine.topn(best_dev, TopN)
-    sorted_topN = [{"angle": angle_comb[i], "vec_score": scores[i], "vec_trans": _trans_tuple(trans[i])}
-                   for i in top]
+    report = {"fp64_recheck": bool(FP64_RECHECK), "fp32_top": [int(i) for i in top]}
+    if FP64_RECHECK and len(top):
+        near = recheck_rows(scores, top)
+        s64, t64 = s.rescore(angle_comb[near])
+        pick = _fp64_top(near, s64, TopN)
+        top = [int(near[k]) for k in pick]
+        report.update(recheck_rows=int(len(near)), fp64_top=top,
+                      rank_changes=int(sum(a != b for a, b in zip(report["fp32_top"], top))),
+                      trans_changes=int(sum(tuple(t64[k]) != tuple(trans[near[k]]) for k in pick)),
+                      max_rel_fp32_fp64=float(np.max(np.abs(s64 - scores[near]) / np.abs(s64))))
+        sorted_topN = [{"angle": angle_comb[near[k]], "vec_score": s64[k], "vec_trans": _trans_tuple(t64[k])}
+                       for k in pick]
+    else:
+        sorted_topN = [{"angle": angle_comb[i], "vec_score": scores[i], "vec_trans": _trans_tuple(trans[i])}
+                       for i in top]
     for i, r in enumerate(sorted_topN):
         new_trans = convert_trans(mrc_target.cent, mrc_search.cent, euler2rot(r["angle"]), r["vec_trans"],
                                   mrc_search.xwidth, mrc_search.xdim)
@@ -274,6 +322,13 @@ def search_map_fft(mrc_target, mrc_search, TopN=10, ang=30, mode="VecProduct", i
         ref_rows = np.concatenate([refine_rows(r["angle"]) for r in sorted_topN], axis=0)
         rs, rt, rbest = s.score(ref_rows)
         rtop = engine.topn(rbest, TopN)
+        if FP64_RECHECK:
+            # every refinement row is decided in FP64
+            rs, rt = s.rescore(ref_rows)
+            report["refine_fp32_top"] = [int(i) for i in rtop]
+            rtop = _fp64_top(np.arange(len(ref_rows)), rs, TopN)
+            report["refine_fp64_top"] = [int(i) for i in rtop]
+            report["refine_rank_changes"] = int(sum(a != b for a, b in zip(report["refine_fp32_top"], rtop)))
         refined_list = []
         for i in rtop:
             od, ov = s.rotated(ref_rows[i])
@@ -308,6 +363,7 @@ def search_map_fft(mrc_target, mrc_search, TopN=10, ang=30, mode="VecProduct", i
                      r["vec_trans"], r["angle"], folder_path, i)
     for r in refined_list:
         r.pop("_dev", None)
+    search_map_fft.last_report = report
     return refined_list
 
 

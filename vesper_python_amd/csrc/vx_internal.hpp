@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stddef.h>
+#include <stdint.h>
 
 #include "vx_fft.hpp"
 
@@ -28,5 +29,26 @@ size_t topn_cap(int nrows, int topn);
 int topn_check(const float* scores, int nrows, int stride, int col, int topn, const int32_t* rows);
 int topn_run(const float* scores, int nrows, int stride, int col, int topn, int32_t* rows,
              unsigned long long* bufA, unsigned long long* bufB, hipStream_t st);
+
+// FP64 re-scoring (vx_rescore.hip).
+struct RsStats { double alpha, vstd, vave, pstd, pave; };
+struct RsInputs {
+    int mode;
+    const double* tdata;
+    const float* tvec;
+    const float* tprob;
+    double ave;
+    const double* sdata;
+    const float* svec;
+    const float* sprob;
+    const float* rot;
+};
+int rs_launch_candidates(const float* maps, int G, int n, int nb, int gout, int r0, const float* best32,
+                         double window, const RsStats& st, int max_cand, int* ncand, int* cand, hipStream_t s);
+int rs_launch_direct(const RsInputs& in, int n, int lo, int w, int fma, int gout, int max_cand, int r0, int nb,
+                     const int* ncand, const int* cand, double* partial, hipStream_t s);
+int rs_launch_select(int nrows, int gout, int n, int max_cand, int prob, const int* ncand, const int* cand,
+                     const double* partial, const RsStats& st, double* score, int32_t* trans, hipStream_t s);
+size_t rs_partial_doubles(int nrows, int gout, int max_cand);
 
 }  // namespace vx

@@ -29,6 +29,7 @@
 #include "vx_common.hpp"
 #include "vx_fft.hpp"
 #include "vx_internal.hpp"
+#include "vx_gather.hpp"
 
 namespace vx {
 
@@ -160,45 +161,6 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
 // ---------------------------------------------------------------------------
 // Rotate-and-gather
 // ---------------------------------------------------------------------------
-
-// Source rows i (per axis) that land on destination index d after
-// int(int32(i - cent) + cent) (VESPER_1_prob_new.py:329, :348), latest first.
-// Even n: the identity.  Odd n (h = (n-1)/2): d < h <- d; d == h <- {h+1, h};
-// h < d < n-1 <- d+1; d == n-1 <- nothing.
-__device__ __forceinline__ int dest_candidates(int d, int n, int* c) {
-    if ((n & 1) == 0) { c[0] = d; return 1; }
-    const int h = (n - 1) >> 1;
-    if (d < h) { c[0] = d; return 1; }
-    if (d == h) { c[0] = h + 1; c[1] = h; return 2; }
-    if (d < n - 1) { c[0] = d + 1; return 1; }
-    return 0;
-}
-
-// old_j = sum_k n_k F[j][k] in the K=3 sgemm order, then + cent (float32).
-__device__ __forceinline__ float rot_coord(float n0, float n1, float n2, const float* Fj, float cent, bool fma) {
-    float t = __fmul_rn(n0, Fj[0]);
-    if (fma) {
-        t = __builtin_fmaf(n1, Fj[1], t);
-        t = __builtin_fmaf(n2, Fj[2], t);
-    } else {
-        t = __fadd_rn(t, __fmul_rn(n1, Fj[1]));
-        t = __fadd_rn(t, __fmul_rn(n2, Fj[2]));
-    }
-    return __fadd_rn(t, cent);
-}
-
-// new_vec_j = sum_k v_k F[k][j]   (VESPER_1_prob_new.py:341)
-__device__ __forceinline__ float rot_vec(float v0, float v1, float v2, const float* F, int j, bool fma) {
-    float t = __fmul_rn(v0, F[j]);
-    if (fma) {
-        t = __builtin_fmaf(v1, F[3 + j], t);
-        t = __builtin_fmaf(v2, F[6 + j], t);
-    } else {
-        t = __fadd_rn(t, __fmul_rn(v1, F[3 + j]));
-        t = __fadd_rn(t, __fmul_rn(v2, F[6 + j]));
-    }
-    return t;
-}
 
 // Gather the value(s) landing on destination (x, y, z) for rotation F.
 //   KIND_VEC:     svec {vx, vy, vz, data != 0}        -> out[0..2]
@@ -2037,13 +1999,24 @@ struct vx_plan {
     int dft_lo, dft_w, dft_kp; // box the F matrices were built for (dft_w = 0: none)
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
+    // FP64 re-scoring workspace (vx_rescore_f64), grown on demand
+    float* rs_maps;        // [batch][G][n^3] float32 correlation maps of a batch
+    size_t rs_maps_cap;
+    float* rs_best;        // [rows][gout] float32 maxima
+    int32_t* rs_trans;     // [rows][gout][3]
+    int* rs_ncand;         // [rows][gout]
+    int* rs_cand;          // [rows][gout][max_cand]
+    double* rs_partial;    // [rows][gout][max_cand][split][2]
+    size_t rs_rows_cap;    // rows x gout x max_cand the buffers above hold
+    int rs_rows_alloc, rs_mc_alloc;
 };
 
 static void plan_free(vx_plan* p) {
     if (!p) return;
     void* ptrs[] = {p->twid, p->svec, p->sscal, p->sprob, p->sdat, p->q, p->qd, p->spec,
                     p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft,
-                    p->q2, p->qd2, p->spec2, p->sum2, p->keys2, p->real_out2};
+                    p->q2, p->qd2, p->spec2, p->sum2, p->keys2, p->real_out2,
+                    p->rs_maps, p->rs_best, p->rs_trans, p->rs_ncand, p->rs_cand, p->rs_partial};
     for (void* x : ptrs)
         if (x) (void)hipFree(x);
     if (p->st2) (void)hipStreamDestroy(p->st2);
@@ -2394,7 +2367,7 @@ struct Workspace {
 // Score rows [r0, r0 + nb) in workspace ws on stream st; ev (7 events) marks
 // the stage boundaries for the profiler.
 static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0, int nb, float* best,
-                       int32_t* trans, hipStream_t st, hipEvent_t* ev) {
+                       int32_t* trans, hipStream_t st, hipEvent_t* ev, float* maps = nullptr) {
     const int n = p->n;
     const size_t n3 = (size_t)n * n * n;
     const int nk = p->nkeys;
@@ -2428,7 +2401,9 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     }
     if (rc != VX_OK) return rc;
     VX_MARK(3);
-    float* ro = (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
+    // maps != nullptr: every group's real correlation map is kept (FP64
+    // re-scoring picks its candidates there); prob mixing reads them too
+    float* ro = maps ? maps : (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
     rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0)
                    : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt,
                                    p->static_plans != 0);
@@ -2437,7 +2412,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     if (p->mode == VX_MODE_PROB && p->mix) {
         const float wa = (float)(1.0 - p->alpha), wb = (float)p->alpha;
         unsigned gx = (unsigned)std::min<size_t>((n3 + 255) / 256, 512);
-        hipLaunchKernelGGL(k_mixed, dim3(gx, nb), dim3(256), 0, st, ws.real_out, n3,
+        hipLaunchKernelGGL(k_mixed, dim3(gx, nb), dim3(256), 0, st, ro, n3,
                            (float)p->vave, (float)p->vstd, (float)p->pave, (float)p->pstd, wa, wb,
                            ws.keys, nk);
         VX_LAUNCH_CHECK();
@@ -2451,11 +2426,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     return VX_OK;
 }
 
-static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
-                           hipStream_t st, hipEvent_t* ev, double* stage_ms) {
-    VX_REQUIRE(p && rot && best && trans, VX_ERR_ARG, "NULL argument");
-    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
-    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+static int clear_if_dirty(vx_plan* p, hipStream_t st) {
     const size_t n3 = (size_t)p->n * p->n * p->n;
     if (p->q_dirty) {
         // the gather writes only the support box: clear the planes once
@@ -2465,6 +2436,16 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
         if (p->qd2) VX_HIP(hipMemsetAsync(p->qd2, 0, (size_t)p->batch * n3 * sizeof(double), st));
         p->q_dirty = false;
     }
+    return VX_OK;
+}
+
+static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
+                           hipStream_t st, hipEvent_t* ev, double* stage_ms) {
+    VX_REQUIRE(p && rot && best && trans, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
+    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+    int rc0 = clear_if_dirty(p, st);
+    if (rc0 != VX_OK) return rc0;
     const Workspace ws0 = {p->q, p->qd, p->spec, p->sum, p->keys, p->real_out};
     const int nbatch = (nrows + p->batch - 1) / p->batch;
     if (ev || p->nstreams < 2 || nbatch < 2) {
@@ -2505,6 +2486,81 @@ static int score_rows_impl(vx_plan* p, const float* rot, int nrows, float* best,
 extern "C" int vx_score_rows(vx_plan* p, const float* rot, int nrows, float* best, int32_t* trans,
                              vx_stream stream) {
     return score_rows_impl(p, rot, nrows, best, trans, (hipStream_t)stream, nullptr, nullptr);
+}
+
+// FP64 re-scoring workspace: grow-only, freed with the plan.
+static int rs_grow(void** ptr, size_t bytes, size_t old_bytes, vx_plan* p) {
+    if (*ptr && bytes <= old_bytes) return VX_OK;
+    if (*ptr) { VX_HIP(hipFree(*ptr)); *ptr = nullptr; }
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        *ptr = nullptr;
+        set_error("device allocation of %zu bytes for the FP64 re-scoring workspace failed", bytes);
+        return VX_ERR_NOMEM;
+    }
+    p->bytes += bytes - (old_bytes < bytes ? old_bytes : 0);
+    return VX_OK;
+}
+
+extern "C" int vx_rescore_f64(vx_plan* p, const double* tdata, const float* tvec, const float* tprob, double t_ave,
+                              const double* sdata, const float* svec, const float* sprob, const float* rot,
+                              int nrows, double window, int max_cand, double* score, int32_t* trans,
+                              int32_t* ncand, vx_stream stream) {
+    VX_REQUIRE(p && rot && score && trans && ncand, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE(p->have_target && p->have_search, VX_ERR_STATE, "target/search not set");
+    VX_REQUIRE(nrows >= 0, VX_ERR_ARG, "negative row count");
+    VX_REQUIRE(window >= 0.0 && window < 1.0, VX_ERR_ARG, "window must be in [0, 1)");
+    VX_REQUIRE(max_cand >= 1 && max_cand <= 4096, VX_ERR_ARG, "max_cand must be in [1, 4096]");
+    VX_REQUIRE(sdata, VX_ERR_ARG, "sdata_dev is required (density masks)");
+    const bool prob = p->mode == VX_MODE_PROB;
+    if (p->mode == VX_MODE_VEC || prob) VX_REQUIRE(tvec && svec, VX_ERR_ARG, "tvec_dev and svec_dev required");
+    if (prob) VX_REQUIRE(tprob && sprob, VX_ERR_ARG, "tprob_dev and sprob_dev required in prob mode");
+    if (p->mode != VX_MODE_VEC && !prob) VX_REQUIRE(tdata, VX_ERR_ARG, "tdata_dev required in this mode");
+    VX_REQUIRE(p->xvariant != 2, VX_ERR_UNSUPPORTED, "FP64 re-scoring runs on the FFT x passes (variant 0, 1 or 3)");
+    if (nrows == 0) return VX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const int n = p->n;
+    const size_t n3 = (size_t)n * n * n;
+    const int gout = prob ? (p->mix ? 3 : 2) : 1;
+    const size_t slots = (size_t)nrows * gout;
+    int rc = rs_grow((void**)&p->rs_maps, (size_t)p->batch * p->G * n3 * sizeof(float),
+                     p->rs_maps_cap * sizeof(float), p);
+    if (rc != VX_OK) return rc;
+    p->rs_maps_cap = std::max(p->rs_maps_cap, (size_t)p->batch * p->G * n3);
+    if (nrows > p->rs_rows_alloc || max_cand > p->rs_mc_alloc) {
+        const int rows = std::max(nrows, p->rs_rows_alloc);
+        const int mc = std::max(max_cand, p->rs_mc_alloc);
+        const size_t sl = (size_t)rows * 3;   // up to 3 groups
+        const size_t osl = (size_t)p->rs_rows_alloc * 3;
+        rc = rs_grow((void**)&p->rs_best, sl * sizeof(float), osl * sizeof(float), p);
+        if (!rc) rc = rs_grow((void**)&p->rs_trans, sl * 3 * sizeof(int32_t), osl * 3 * sizeof(int32_t), p);
+        if (!rc) rc = rs_grow((void**)&p->rs_cand, sl * mc * sizeof(int), osl * p->rs_mc_alloc * sizeof(int), p);
+        if (!rc) rc = rs_grow((void**)&p->rs_partial, rs_partial_doubles(rows, 3, mc) * sizeof(double),
+                              rs_partial_doubles(p->rs_rows_alloc, 3, p->rs_mc_alloc) * sizeof(double), p);
+        if (rc != VX_OK) return rc;
+        p->rs_rows_alloc = rows;
+        p->rs_mc_alloc = mc;
+    }
+    rc = clear_if_dirty(p, st);
+    if (rc != VX_OK) return rc;
+    VX_HIP(hipMemsetAsync(ncand, 0, slots * sizeof(int32_t), st));
+    const Workspace ws0 = {p->q, p->qd, p->spec, p->sum, p->keys, p->real_out};
+    const RsStats stats = {p->alpha, p->vstd, p->vave, p->pstd, p->pave};
+    const RsInputs in = {p->mode, tdata, tvec, tprob, t_ave, sdata, svec, sprob, rot};
+    const int fma = (p->flags & VX_FLAG_NO_FMA) ? 0 : 1;
+    for (int r0 = 0; r0 < nrows; r0 += p->batch) {
+        const int nb = std::min(p->batch, nrows - r0);
+        rc = score_batch(p, ws0, rot, r0, nb, p->rs_best, p->rs_trans, st, nullptr, p->rs_maps);
+        if (rc != VX_OK) return rc;
+        rc = rs_launch_candidates(p->rs_maps, p->G, n, nb, gout, r0, p->rs_best, window, stats, max_cand, ncand,
+                                  p->rs_cand, st);
+        if (rc != VX_OK) return rc;
+        rc = rs_launch_direct(in, n, p->lo, p->w, fma, gout, max_cand, r0, nb, ncand, p->rs_cand, p->rs_partial,
+                              st);
+        if (rc != VX_OK) return rc;
+    }
+    return rs_launch_select(nrows, gout, n, max_cand, prob ? 1 : 0, ncand, p->rs_cand, p->rs_partial, stats,
+                            score, trans, st);
 }
 
 // Per-channel correlation maps against caller-held conjugated spectra:

@@ -279,6 +279,37 @@ class SearchPlan:
             nat.call("vx_score_rows", self._h, _p(rot), R, _p(best), _p(trans), _stream())
         return best, trans
 
+    def rescore_f64(self, rot: torch.Tensor, target, search, ave: float = 0.0, window: float = 1e-5,
+                    max_cand: int = 64):
+        """FP64 scores and translations of the rows of rot [R, 9] (device
+        float32), decided among the translations within `window` (relative)
+        of each row's float32 maximum (vx_rescore_f64).  target / search =
+        (data float64, vec float32, prob float32 [4, n, n, n] or None): the
+        maps of set_target / set_search.  Returns (score [R, G] float64, trans
+        [R, G, 3] int32, ncand [R, G] int32) device tensors.  A row with more
+        candidates than max_cand (a plateau of equal values) is re-run with
+        room for all of them, up to 4096; beyond that it raises."""
+        R = int(rot.shape[0])
+        G = self.groups
+        score = torch.empty((R, G), dtype=torch.float64, device=rot.device)
+        trans = torch.empty((R, G, 3), dtype=torch.int32, device=rot.device)
+        ncand = torch.zeros((R, G), dtype=torch.int32, device=rot.device)
+        if R:
+            td, tv, tp = (tuple(target) + (None, None, None))[:3]
+            sd, sv, sp = (tuple(search) + (None, None, None))[:3]
+            mc = int(max_cand)
+            while True:
+                nat.call("vx_rescore_f64", self._h, _p(td), _p(tv), _p(tp), float(ave), _p(sd), _p(sv), _p(sp),
+                         _p(rot), R, float(window), mc, _p(score), _p(trans), _p(ncand), _stream())
+                worst = int(ncand.max().item())
+                if worst <= mc:
+                    break
+                if mc >= 4096:
+                    raise VesperError(f"FP64 re-scoring: a row has {worst} translations within the window "
+                                      f"{window} of its maximum (at most 4096)")
+                mc = min(4096, 1 << (worst - 1).bit_length())
+        return score, trans, ncand
+
     STAGES = ("gather", "fwd_zy", "xcorr", "inv_yz", "mixed", "finalize")
 
     def profile_rows(self, rot: torch.Tensor) -> dict:

@@ -47,6 +47,14 @@ class ProbSearch:
         best, trans = self.plan.score_rows(rot)
         return best, trans
 
+    def rescore(self, angles):
+        """FP64 (raw scores [R, G], trans [R, G, 3]) of the rows: vector,
+        probability and (once mixing is on) mixed groups (vx_rescore_f64)."""
+        rot = torch.from_numpy(rotation_rows(angles)).to(self.tdata.device)
+        sc, tr, _ = self.plan.rescore_f64(rot, (self.tdata, self.tvec, self.tprob),
+                                          (self.sdata, self.svec, self.sprob))
+        return sc.cpu().numpy(), tr.cpu().numpy()
+
     def rotated(self, angle):
         od, ov, op = engine.rotate_map(rotation_rows(angle)[0], self.sdata, self.svec, self.sprob)
         return od, ov, op
@@ -89,7 +97,19 @@ def run_prob_search(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, mrc_
     top = engine.topn(best.reshape(-1), TopN, stride=3, col=2)
     bh = best.cpu().numpy()
     th = trans.cpu().numpy()
-    sorted_topN = [_row(angle_comb[i], bh[i], th[i], s.rd3, True) for i in top]
+    from . import VESPER_1_prob_new as V
+    report = {"fp64_recheck": bool(V.FP64_RECHECK), "fp32_top": [int(i) for i in top]}
+    if V.FP64_RECHECK and len(top):
+        # the rows near the cutoff of the mixed-score ranking, decided in FP64
+        near = V.recheck_rows(bh[:, 2].astype(np.float64), top)
+        b64, t64 = s.rescore(angle_comb[near])
+        pick = V._fp64_top(near, b64[:, 2], TopN)
+        top = [int(near[k]) for k in pick]
+        report.update(recheck_rows=int(len(near)), fp64_top=top,
+                      rank_changes=int(sum(a != b for a, b in zip(report["fp32_top"], top))))
+        sorted_topN = [_row(angle_comb[near[k]], b64[k], t64[k], s.rd3, True) for k in pick]
+    else:
+        sorted_topN = [_row(angle_comb[i], bh[i], th[i], s.rd3, True) for i in top]
     # the reference prints the row dicts under numpy 1.21 (requirements.txt),
     # whose scalar reprs carry no type name
     with np.printoptions(legacy="1.25"):
@@ -103,6 +123,12 @@ def run_prob_search(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, mrc_
         rtop = engine.topn(rbest.reshape(-1), TopN, stride=3, col=2)
         rb = rbest.cpu().numpy()
         rt = rtrans.cpu().numpy()
+        if V.FP64_RECHECK:
+            # every refinement row is decided in FP64
+            rb, rt = s.rescore(ref_rows)
+            report["refine_fp32_top"] = [int(i) for i in rtop]
+            rtop = V._fp64_top(np.arange(len(ref_rows)), rb[:, 2], TopN)
+            report["refine_fp64_top"] = [int(i) for i in rtop]
         refined_list = [_row(tuple(ref_rows[i]), rb[i], rt[i], s.rd3, True) for i in rtop]
     else:
         refined_list = sorted_topN
@@ -132,4 +158,5 @@ def run_prob_search(mrc_P1, mrc_P2, mrc_P3, mrc_P4, mrc_target, mrc_search, mrc_
                      r["mixed_trans"], r["angle"], folder_path, i)
     for r in refined_list:
         r.pop("_dev", None)
+    run_prob_search.last_report = report
     return refined_list
