@@ -120,6 +120,9 @@ int vx_plan_destroy(vx_plan* plan);
                                   every stage's radix, span and twiddle offset as constants;
                                   0 = runtime descriptors only.  Results equal to fp32
                                   rounding (bit-identical at 175). */
+#define VX_TUNE_RADIX_ORDER 9 /* order of the FFT stages: 0 (default) = radix 8, 4, 2 then the odd
+                                 primes ascending; 1 = reversed; 2 = odd primes ascending first;
+                                 3 = odd primes descending first.  Results equal to fp32 rounding. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

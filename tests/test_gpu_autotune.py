@@ -41,9 +41,35 @@ def test_autotune_applies_a_valid_schedule(gpu, n, batch):
     assert "xpass_variant" in sched and sched.get("streams") in (1, 2)
     b1, t1 = plan.score_rows(rot)
     b1, t1 = b1.cpu().numpy(), t1.cpu().numpy()
-    if sched["xpass_variant"] == 2 or sched.get("pencil"):
+    if sched["xpass_variant"] == 2 or sched.get("pencil") or sched.get("radix_order"):
         # held to the autotuner's 1e-4 gate
         assert np.max(np.abs(b1 - b0)) <= 1e-4 * np.max(np.abs(b0))
     else:
         assert np.array_equal(b1, b0) and np.array_equal(t1, t0)
+    plan.close()
+
+
+@pytest.mark.parametrize("n,static", [(175, True), (96, True), (96, False)])
+def test_radix_orders_agree(gpu, n, static):
+    """Every stage order of the FFT (VX_TUNE_RADIX_ORDER) scores the rows
+    within fp32 rounding of the default order: at 175 = 5 5 7 and 7 5 5, both
+    compiled; at 96 = 8 4 3, 3 4 8 and 3 8 4 on compile-time or runtime plans."""
+    plan, rot = _plan(n, 400 + n, 8)
+    plan.tune(static_plans=static)
+    ref_b, ref_t = plan.score_rows(rot)
+    ref_b, ref_t = ref_b.double().cpu().numpy(), ref_t.cpu().numpy()
+    orders = autotune.radix_orders(n)
+    assert len(orders) >= 2
+    for order, radices in orders.items():
+        print("[case] radix order", n, order, radices, flush=True)
+        plan.tune(radix_order=order)
+        b, t = plan.score_rows(rot)
+        b, t = b.double().cpu().numpy(), t.cpu().numpy()
+        rel = np.max(np.abs(b - ref_b)) / np.max(np.abs(ref_b))
+        print("[radix]", n, radices, "max rel", rel, "translations equal", np.mean(np.all(t == ref_t, axis=-1)),
+              flush=True)
+        assert rel <= 1e-5
+    plan.tune(radix_order=0)
+    b, t = plan.score_rows(rot)
+    assert np.array_equal(b.double().cpu().numpy(), ref_b) and np.array_equal(t.cpu().numpy(), ref_t)
     plan.close()

@@ -1,7 +1,9 @@
 """Autotuner for the search kernels' schedules.
 
 For a plan (grid edge N, mode, batch), times every candidate schedule of the
-x pass (register- or LDS-accumulating kernel, tile width) on a sample of the
+x pass (register- or LDS-accumulating kernel, tile width), the slab shape or
+pencil passes, the order of the FFT's radix stages and the number of batches
+in flight on a sample of the
 actual rotation rows with the library's per-stage HIP-event profiler
 (vx_score_rows_profile), keeps the fastest, and caches it per
 (device, N, mode, batch) in a JSON file.  The FFT schedules (variants 0, 1)
@@ -37,6 +39,28 @@ def candidates(n: int, groups: int = 1):
     if groups == 1 and n <= 192:
         for t in (32, 64):
             out.append(dict(xpass_variant=2, xpass_tile=t))
+    return out
+
+
+def radix_orders(n: int) -> dict:
+    """The distinct FFT stage orders of grid edge n that the library's
+    VX_TUNE_RADIX_ORDER values 0..3 give: {value: radices}."""
+    two, odd, m = [], [], n
+    for r in (8, 4, 2):
+        while m % r == 0:
+            two.append(r)
+            m //= r
+    for q in (3, 5, 7, 11, 13):
+        while m % q == 0:
+            odd.append(q)
+            m //= q
+    orders = {0: two + odd, 1: (two + odd)[::-1], 2: odd + two, 3: odd[::-1] + two}
+    out, seen = {}, set()
+    for k in range(4):
+        t = tuple(orders[k])
+        if t not in seen:
+            seen.add(t)
+            out[k] = t
     return out
 
 
@@ -78,6 +102,7 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
     sample = rot[:nrows]
     results = []
     exact = None
+    plan.tune(radix_order=0)
     for cand in candidates(plan.n, plan.groups):
         try:
             plan.tune(**cand)
@@ -134,6 +159,30 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
             plan.tune(pencil=False)
     if "slab_threads" not in best:
         best = dict(best, slab_threads=0)
+    # the order of the FFT stages (equal to fp32 rounding: held to the 1e-4 gate)
+    best = dict(best, radix_order=0)
+    for order, radices in radix_orders(plan.n).items():
+        if order == 0:
+            continue
+        cand = dict(best, radix_order=order)
+        try:
+            plan.tune(**cand)
+            got, _ = plan.score_rows(sample[:plan.batch])
+            ms2 = plan.profile_rows(sample)
+        except Exception as e:
+            if verbose:
+                print("[tune] skip", cand, e, file=sys.stderr)
+            continue
+        got = got.double().cpu().numpy()
+        t2 = sum(ms2.values())
+        if verbose:
+            print(f"[tune] radices {radices} total={t2:.3f} ms", file=sys.stderr, flush=True)
+        if exact is None or np.max(np.abs(got - exact)) > APPROX_RTOL * max(np.max(np.abs(exact)), 1e-30):
+            continue
+        results.append((t2, cand, ms2))
+        if t2 < total:
+            total, best, ms = t2, cand, ms2
+    plan.tune(radix_order=best["radix_order"])
     # two batches in flight: only a wall-clock measurement shows the overlap
     if nrows >= 2 * plan.batch:
         walls = {}

@@ -44,6 +44,7 @@ struct FftDesc {
     int nst;
     int ntw;                     // twiddle table length (float2)
     unsigned radix_mask;         // bit r set when a radix-r stage occurs
+    unsigned long long rcode;    // stage radices in order, 4 bits each (stage 0 lowest)
     const FftStage* st;          // device [nst]
     const float2* twid;          // twiddle table: device, or LDS once staged
 };
@@ -468,11 +469,22 @@ __device__ __forceinline__ void slab_fft_static_rec(float2* buf, const float2* t
 template <int NL, int... Rs>
 struct StaticTile;
 
+// Stage radices in order, 4 bits each, stage 0 in the lowest bits (the
+// descriptor's rcode).
+template <int... Rs>
+constexpr unsigned long long radix_code() {
+    unsigned long long c = 0;
+    int s = 0;
+    ((c |= (unsigned long long)Rs << (4 * s++)), ...);
+    return c;
+}
+
 template <int... Rs>
 struct StaticPlan {
     static constexpr bool kStatic = true;
     static constexpr int kN = (Rs * ...);
     static constexpr unsigned kMask = (radix_bit(Rs) | ...);
+    static constexpr unsigned long long kCode = radix_code<Rs...>();
     // the same radices as a column-tile plan of NL lines
     template <int NL>
     using Tile = StaticTile<NL, Rs...>;

@@ -17,7 +17,7 @@ struct RotF { float m[9]; };
 // Gather kinds (what a destination voxel collects from the source map).
 enum { KIND_VEC = 0, KIND_SCALAR = 1, KIND_PROB = 2, KIND_DATA64 = 3 };
 
-int fft_desc_create(int n, FftDesc* d, float2** twid_out);
+int fft_desc_create(int n, FftDesc* d, float2** twid_out, int order = 0);
 int launch_gather(int kind, const float4* svec, const float2* sscal, const float4* sprob,
                   const double* sdat, const float* rot, int n, int nb, int fma, float* q,
                   double* qd, unsigned long long* keys, int nkeys, int lo, int w, int slo, int sw,

@@ -37,24 +37,37 @@ namespace vx {
 // FFT descriptor construction
 // ---------------------------------------------------------------------------
 
-static bool factorize(int n, std::vector<int>& r) {
+// Stage radices of a length n: powers of two in radix 8, 4, 2, then the odd
+// primes.  `order` permutes the stages (the autotuner's radix-order knob):
+// 0 = that order; 1 = reversed; 2 = the odd primes first (ascending), then the
+// powers of two; 3 = the odd primes first, descending, then the powers of two.
+static bool factorize(int n, std::vector<int>& r, int order = 0) {
     r.clear();
     int m = n;
-    while (m % 8 == 0) { r.push_back(8); m /= 8; }
-    while (m % 4 == 0) { r.push_back(4); m /= 4; }
-    while (m % 2 == 0) { r.push_back(2); m /= 2; }
-    const int odd[] = {3, 5, 7, 11, 13};
-    for (int p : odd)
-        while (m % p == 0) { r.push_back(p); m /= p; }
+    std::vector<int> two, odd;
+    while (m % 8 == 0) { two.push_back(8); m /= 8; }
+    while (m % 4 == 0) { two.push_back(4); m /= 4; }
+    while (m % 2 == 0) { two.push_back(2); m /= 2; }
+    const int primes[] = {3, 5, 7, 11, 13};
+    for (int p : primes)
+        while (m % p == 0) { odd.push_back(p); m /= p; }
+    if (order == 3) std::reverse(odd.begin(), odd.end());
+    if (order >= 2) {
+        r = odd;
+        r.insert(r.end(), two.begin(), two.end());
+    } else {
+        r = two;
+        r.insert(r.end(), odd.begin(), odd.end());
+        if (order == 1) std::reverse(r.begin(), r.end());
+    }
     return m == 1 && (int)r.size() <= kMaxStages;
 }
 
-// True when fft_desc_create's stage order for n is exactly SP's radices (the
-// compile-time plan then reads the descriptor's twiddle table unchanged).
+// True when the descriptor's stages are exactly SP's radices in SP's order
+// (the compile-time plan then reads the descriptor's twiddle table unchanged).
 template <class SP>
-static bool plan_matches(int n) {
-    std::vector<int> r;
-    return n == SP::kN && factorize(n, r) && SP::same_radices(r.data(), (int)r.size());
+static bool plan_matches(const FftDesc& fd) {
+    return fd.n == SP::kN && fd.rcode == SP::kCode;
 }
 
 // Grid edges with compile-time stage plans (radices in fft_desc_create's
@@ -72,7 +85,8 @@ using StaticLengths = PlanList<StaticPlan<2, 3, 3, 5>, StaticPlan<7, 13>, Static
     StaticPlan<2, 5, 13>, StaticPlan<4, 3, 11>, StaticPlan<3, 3, 3, 5>, StaticPlan<4, 5, 7>,
     StaticPlan<8, 2, 3, 3>, StaticPlan<3, 7, 7>, StaticPlan<2, 3, 5, 5>, StaticPlan<2, 7, 11>,
     StaticPlan<4, 3, 13>, StaticPlan<8, 4, 5>, StaticPlan<2, 3, 3, 3, 3>, StaticPlan<3, 5, 11>,
-    StaticPlan<8, 3, 7>, StaticPlan<5, 5, 7>, StaticPlan<8, 2, 11>, StaticPlan<4, 3, 3, 5>>;
+    StaticPlan<8, 3, 7>, StaticPlan<5, 5, 7>, StaticPlan<8, 2, 11>, StaticPlan<4, 3, 3, 5>,
+    StaticPlan<7, 5, 5>>;   // 175 with its radix-7 stage first (radix orders 1 and 3)
 
 // The FFTW-fast lengths from 16 to 89 with two stages or more: their slab
 // takes the 8 x 512 workgroup shape.
@@ -111,21 +125,22 @@ using LargeLengths = PlanList<StaticPlan<4, 5, 13>, StaticPlan<8, 3, 11>, Static
     StaticPlan<3, 5, 5, 5>, StaticPlan<2, 3, 3, 3, 7>, StaticPlan<8, 8, 2, 3>, StaticPlan<5, 7, 11>,
     StaticPlan<2, 3, 5, 13>, StaticPlan<8, 7, 7>, StaticPlan<4, 3, 3, 11>, StaticPlan<8, 2, 5, 5>>;
 
-// Calls f(SP{}) for the listed plan whose length is n; false if none is.
+// Calls f(SP{}) for the listed plan whose stages are the descriptor's; false if none is.
 template <class F, class... Ps>
-static bool with_static_plan(int n, PlanList<Ps...>, F&& f) {
+static bool with_static_plan(const FftDesc& fd, PlanList<Ps...>, F&& f) {
     bool hit = false;
-    ((hit = hit || (plan_matches<Ps>(n) && (f(Ps{}), true))), ...);
+    ((hit = hit || (plan_matches<Ps>(fd) && (f(Ps{}), true))), ...);
     return hit;
 }
 
-int fft_desc_create(int n, FftDesc* d, float2** twid_out) {
+int fft_desc_create(int n, FftDesc* d, float2** twid_out, int order) {
     std::vector<int> r;
-    VX_REQUIRE(n >= 2 && factorize(n, r), VX_ERR_UNSUPPORTED,
+    VX_REQUIRE(n >= 2 && factorize(n, r, order), VX_ERR_UNSUPPORTED,
                "grid edge %d is not 2^a 3^b 5^c 7^d 11^e 13^f", n);
     memset(d, 0, sizeof(*d));
     d->n = n;
     d->nst = (int)r.size();
+    for (int s = 0; s < d->nst; ++s) d->rcode |= (unsigned long long)r[s] << (4 * s);
     std::vector<FftStage> stages(d->nst);
     std::vector<float2> tw;
     int p = 1;
@@ -1523,9 +1538,9 @@ static int launch_axis_y(float2* buf, const FftDesc& fd, int n, int inverse, int
     };
     bool hit = false;
     if (use_static && full > 0 && T == 8)
-        hit = with_static_plan(n, TileLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 8>{}); });
+        hit = with_static_plan(fd, TileLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 8>{}); });
     else if (use_static && full > 0 && T == 4)
-        hit = with_static_plan(n, LargeLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 4>{}); });
+        hit = with_static_plan(fd, LargeLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 4>{}); });
     VX_LAUNCH_CHECK();
     const int t0 = hit ? full : 0;
     if (t0 < ntile) {
@@ -1553,7 +1568,7 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
         hipLaunchKernelGGL(k_fwd_z<SP>, gz, dim3(256), l, st, q, spec, fd, n, nhz, lo, w, P);
     };
     if (!(use_static && n * P <= 256 * kPencilMaxV &&
-          (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
+          (with_static_plan(fd, TileLengths{}, zgo) || with_static_plan(fd, LargeLengths{}, zgo))))
         zgo(DynamicPlan{});
     VX_LAUNCH_CHECK();
     return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st);
@@ -1576,7 +1591,7 @@ int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, uns
         hipLaunchKernelGGL(k_inv_z<SP>, gz, dim3(256), l, st, sum, fd, n, nhz, G, P, keys, nkeys, real_out);
     };
     if (!(use_static && n * P <= 256 * kPencilMaxV &&
-          (with_static_plan(n, TileLengths{}, zgo) || with_static_plan(n, LargeLengths{}, zgo))))
+          (with_static_plan(fd, TileLengths{}, zgo) || with_static_plan(fd, LargeLengths{}, zgo))))
         zgo(DynamicPlan{});
     VX_LAUNCH_CHECK();
     return VX_OK;
@@ -1688,12 +1703,12 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(w, nplanes);
-    if (maxv == 8 && use_static && with_static_plan(n, SmallLengths{}, [&](auto sp) {
+    if (maxv == 8 && use_static && with_static_plan(fd, SmallLengths{}, [&](auto sp) {
             using SP = decltype(sp);
             VX_SLAB_LAUNCH_SP(k_fwd_zy, 8, 512, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
         })) {
     } else if (maxv == 8) VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
-    else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
+    else if (maxv == 17 && use_static && with_static_plan(fd, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
              })) {
@@ -1702,7 +1717,7 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     else if (maxv == 17) VX_SLAB_LAUNCH(k_fwd_zy, 16, 1024, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_fwd_zy, 16, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
-    else if (maxv == 24 && use_static && with_static_plan(n, MediumLengths{}, [&](auto sp) {
+    else if (maxv == 24 && use_static && with_static_plan(fd, MediumLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_fwd_zy, 24, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
              })) {
@@ -1720,12 +1735,12 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
     dim3 grid(n, nbg);
-    if (maxv == 8 && use_static && with_static_plan(n, SmallLengths{}, [&](auto sp) {
+    if (maxv == 8 && use_static && with_static_plan(fd, SmallLengths{}, [&](auto sp) {
             using SP = decltype(sp);
             VX_SLAB_LAUNCH_SP(k_inv_yz, 8, 512, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
         })) {
     } else if (maxv == 8) VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
-    else if (maxv == 17 && use_static && with_static_plan(n, StaticLengths{}, [&](auto sp) {
+    else if (maxv == 17 && use_static && with_static_plan(fd, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
              })) {
@@ -1734,7 +1749,7 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     else if (maxv == 17) VX_SLAB_LAUNCH(k_inv_yz, 16, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 16) VX_SLAB_LAUNCH(k_inv_yz, 16, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
-    else if (maxv == 24 && use_static && with_static_plan(n, MediumLengths{}, [&](auto sp) {
+    else if (maxv == 24 && use_static && with_static_plan(fd, MediumLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
                  VX_SLAB_LAUNCH_SP(k_inv_yz, 24, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
              })) {
@@ -1842,10 +1857,10 @@ static void launch_reg_pf(dim3 grid, size_t lds, hipStream_t st, const float2* s
     auto tiles4 = [&](auto sp) { launch_static(sp, std::integral_constant<int, 4>{}); };
     bool hit = false;
     if (use_static && full > 0 && T == 8)
-        hit = with_static_plan(n, StaticLengths{}, tiles8) || with_static_plan(n, SmallLengths{}, tiles8) ||
-              with_static_plan(n, MediumLengths{}, tiles8) || with_static_plan(n, TileLengths{}, tiles8);
+        hit = with_static_plan(fd, StaticLengths{}, tiles8) || with_static_plan(fd, SmallLengths{}, tiles8) ||
+              with_static_plan(fd, MediumLengths{}, tiles8) || with_static_plan(fd, TileLengths{}, tiles8);
     else if (use_static && full > 0 && T == 4 && n > 256)
-        hit = with_static_plan(n, LargeLengths{}, tiles4);
+        hit = with_static_plan(fd, LargeLengths{}, tiles4);
     if (hit) {
         if (full < ntiles)
             launch_reg_pf_dyn<G, RM>(xcorr_grid(ntiles - full, nb), lds, st, spec, tgt, out, fd, n, nhz, nb, C, T,
@@ -2325,6 +2340,18 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
                        "slab threads must be 0, 512 or 1024");
             p->slab_nt = value;
             break;
+        case VX_TUNE_RADIX_ORDER: {
+            VX_REQUIRE(value >= 0 && value <= 3, VX_ERR_ARG, "radix order must be 0, 1, 2 or 3");
+            FftDesc nd;
+            float2* ntw = nullptr;
+            int rc = fft_desc_create(p->n, &nd, &ntw, value);
+            if (rc != VX_OK) return rc;
+            VX_HIP(hipDeviceSynchronize());   // no launch in flight may still read the old stage table
+            (void)hipFree(p->twid);
+            p->fd = nd;
+            p->twid = ntw;
+            break;
+        }
         case VX_TUNE_FORCE_PENCIL:
             if (value) p->pencil = 1;
             else {
