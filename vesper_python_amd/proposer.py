@@ -274,14 +274,39 @@ def neighbour_rows(rows, nx, nz):
     return np.unique(np.concatenate(out))
 
 
+def nearest_coarse_rows(angles, coarse_step):
+    """For each Euler row, the calc_angle_comb row of the coarse grid (step
+    `coarse_step`) nearest to it (x and y wrap at 360, z is clamped to
+    [0, 180]), and its distance in degrees."""
+    a = np.asarray(angles, dtype=np.float64)
+    nx = len(np.arange(0, 360, coarse_step))
+    nz = len(np.arange(0, 180 + 1e-9, coarse_step))
+    ix = np.rint(a[:, 0] / coarse_step).astype(np.int64) % nx
+    iy = np.rint(a[:, 1] / coarse_step).astype(np.int64) % nx
+    iz = np.clip(np.rint(a[:, 2] / coarse_step).astype(np.int64), 0, nz - 1)
+    d = np.stack([a[:, 0] - ix * coarse_step, a[:, 1] - iy * coarse_step, a[:, 2] - iz * coarse_step], axis=1)
+    d = (d + 180.0) % 360.0 - 180.0
+    return iz * nx * nx + ix * nx + iy, np.sqrt((d * d).sum(axis=1))
+
+
+POLICIES = ("mlp", "coarse", "random")
+
+
 def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, topn=10, epochs=40, lr=2e-3,
-                   seed=0, sweeps=3):
+                   seed=0, sweeps=3, policy="mlp", coarse_step=None):
     """Coarse landscape -> train -> propose a share of the rows -> FFT-score
     them -> retrain on everything scored so far -> propose the next share of
     the rows not yet scored ... (`rounds` shares of frac / rounds each); then
     up to `sweeps` passes that FFT-score the unscored grid neighbours of the
     current top rows (hill climbing on the exact scores).
+    `policy` picks the shares: "mlp" (the network's predictions), or, for the
+    ablation with the same budget and sweeps, "coarse" (each fine row ranked
+    by the coarse landscape's score of its nearest coarse row, nearer rows
+    first; coarse_angles must then be the calc_angle_comb grid of step
+    `coarse_step`) or "random" (rows drawn uniformly).
     Returns (top rows into angles, number of fine rows FFT-scored, coarse rows)."""
+    if policy not in POLICIES:
+        raise ValueError(f"unknown policy {policy!r}")
     dev = pr.td.device
     crot, cX = pr.rows_features(coarse_angles)
     cs = landscape(pr.plan, crot).double()
@@ -293,12 +318,22 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
     best_all = torch.full((R,), -float("inf"), dtype=torch.float32, device=dev)
     trainX, trainy = [cX[:int(crot.shape[0])]], [cy]
     share = max(topn, int(math.ceil(frac * R / rounds)))
+    static_pred = None
+    if policy == "coarse":
+        near, dist = nearest_coarse_rows(angles, coarse_step)
+        base = cy.double().cpu().numpy()[near]
+        static_pred = torch.from_numpy((base - 1e-6 * dist).astype(np.float32)).to(dev)
+    elif policy == "random":
+        static_pred = torch.from_numpy(np.random.default_rng(seed).random(R).astype(np.float32)).to(dev)
     for r in range(rounds):
-        Xt = torch.cat(trainX)
-        yt = torch.cat(trainy)
-        pr.mlp.fit(Xt, yt, int(yt.numel()), epochs=epochs if r == 0 else max(5, epochs // 4), batch=512, lr=lr,
-                   seed=seed + r)
-        pred = pr.mlp.forward(X, R).clone()
+        if static_pred is None:
+            Xt = torch.cat(trainX)
+            yt = torch.cat(trainy)
+            pr.mlp.fit(Xt, yt, int(yt.numel()), epochs=epochs if r == 0 else max(5, epochs // 4), batch=512,
+                       lr=lr, seed=seed + r)
+            pred = pr.mlp.forward(X, R).clone()
+        else:
+            pred = static_pred.clone()
         if bool(scored.any()):
             # rows already scored drop just below the lowest unscored prediction
             # (finite, so the selection histogram keeps its resolution)
