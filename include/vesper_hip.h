@@ -123,6 +123,11 @@ int vx_plan_destroy(vx_plan* plan);
 #define VX_TUNE_RADIX_ORDER 9 /* order of the FFT stages: 0 (default) = radix 8, 4, 2 then the odd
                                  primes ascending; 1 = reversed; 2 = odd primes ascending first;
                                  3 = odd primes descending first.  Results equal to fp32 rounding. */
+#define VX_TUNE_SLAB_PERSIST 10 /* 1: the forward and inverse slab passes run as about one workgroup
+                                   per CU, each walking slabs with the next slab's loads in flight
+                                   during the current one's transforms (compile-time plans,
+                                   1024-thread slabs); 0 = one workgroup per slab (default).
+                                   Identical results. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,
@@ -381,9 +386,30 @@ int vx_mlp_features(int M, int ld, const float* rot_dev, const float* quat_dev, 
                     const float* qdesc_dev, void* X_dev, vx_stream stream);
 
 /* Rows whose prediction ranks in about the top k (4096-bin histogram
- * threshold; ties at the threshold bin are all kept).  rows_dev [M];
- * *nsel_host = number written (>= k unless M < k).  Synchronises. */
+ * threshold; ties at the threshold bin are all kept), in ascending order.
+ * rows_dev [M]; *nsel_host = number written (>= k unless M < k).
+ * Synchronises. */
 int vx_select_top(int M, const float* pred_dev, int k, int* rows_dev, int* nsel_host, vx_stream stream);
+
+/* Proposal bookkeeping (device-side; no counterpart in the reference). */
+/* out = (in - mean) / (std + 1e-30), std with n - 1; stats_dev[2] = {mean,
+ * std + 1e-30} computed in FP64 in a fixed order (use_given = 0) or read
+ * (use_given = 1).  in and out may alias. */
+int vx_zscore_f32(int n, const float* in_dev, double* stats_dev, int use_given, float* out_dev, vx_stream stream);
+/* dst row r = src row idx[r] for r < nvalid, zero rows for nvalid <= r < nout;
+ * rows of row_bytes bytes (a multiple of 4). */
+int vx_gather_rows(int nout, int nvalid, const int32_t* idx_dev, const void* src_dev, int row_bytes, void* dst_dev,
+                   vx_stream stream);
+/* pred[i] for scored[i] != 0 set just below the lowest unscored prediction. */
+int vx_mask_scored(int n, float* pred_dev, const uint8_t* scored_dev, vx_stream stream);
+/* out = the rows[i] with scored[rows[i]] == 0, in order; *count_host their
+ * number.  Synchronises. */
+int vx_filter_unscored(int n, const int32_t* rows_dev, const uint8_t* scored_dev, int32_t* out_dev, int* count_host,
+                       vx_stream stream);
+/* dst[idx[i]] = src[i * stride]; flag[idx[i]] = 1 when flag_dev != NULL. */
+int vx_scatter_scores(int n, const int32_t* idx_dev, const float* src_dev, int stride, float* dst_dev,
+                      uint8_t* flag_dev, vx_stream stream);
+int vx_fill_f32(int n, float value, float* dst_dev, vx_stream stream);
 
 #ifdef __cplusplus
 }

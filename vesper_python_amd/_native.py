@@ -24,6 +24,7 @@ VX_TUNE_SLAB_THREADS = 6
 VX_TUNE_STREAMS = 7
 VX_TUNE_STATIC_PLANS = 8
 VX_TUNE_RADIX_ORDER = 9
+VX_TUNE_SLAB_PERSIST = 10
 ABI_VERSION = 1
 
 MODE_CODES = {"VecProduct": VX_MODE_VEC, "Overlap": VX_MODE_OVERLAP, "CC": VX_MODE_CC,
@@ -73,6 +74,12 @@ SIGNATURES = {
     "vx_mlp_descriptors": (_I, [_I, _P, _P, _P, _P]),
     "vx_mlp_features": (_I, [_I, _I, _P, _P, _P, _P, _P, _P]),
     "vx_select_top": (_I, [_I, _P, _I, _P, _P, _P]),
+    "vx_zscore_f32": (_I, [_I, _P, _P, _I, _P, _P]),
+    "vx_gather_rows": (_I, [_I, _I, _P, _P, _I, _P, _P]),
+    "vx_mask_scored": (_I, [_I, _P, _P, _P]),
+    "vx_filter_unscored": (_I, [_I, _P, _P, _P, _P, _P]),
+    "vx_scatter_scores": (_I, [_I, _P, _P, _I, _P, _P, _P]),
+    "vx_fill_f32": (_I, [_I, ctypes.c_float, _P, _P]),
 }
 
 

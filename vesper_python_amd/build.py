@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libvesper_hip.so")
 SOURCES = ["vx_search.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip",
-           "vx_noalloc.hip", "vx_rescore.hip"]
+           "vx_noalloc.hip", "vx_rescore.hip", "vx_propose.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VX_OFFLOAD_ARCH", "gfx950")

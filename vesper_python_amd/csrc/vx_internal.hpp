@@ -51,4 +51,9 @@ int rs_launch_select(int nrows, int gout, int n, int max_cand, int prob, const i
                      const double* partial, const RsStats& st, double* score, int32_t* trans, hipStream_t s);
 size_t rs_partial_doubles(int nrows, int gout, int max_cand);
 
+// Ordered compaction of the rows whose prediction falls in histogram bins >=
+// bin (vx_propose.hip); scratch holds 257 unsigned.
+int compact_top_bins(int n, const float* v, const unsigned* mm, int bin, int32_t* out, unsigned* scratch,
+                     hipStream_t st);
+
 }  // namespace vx

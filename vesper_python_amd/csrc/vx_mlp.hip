@@ -19,6 +19,7 @@
 #include <math.h>
 
 #include "vx_common.hpp"
+#include "vx_internal.hpp"
 
 namespace vx {
 
@@ -457,17 +458,6 @@ __global__ void k_hist(int M, const float* __restrict__ v, const unsigned* __res
     }
 }
 
-__global__ void k_compact(int M, const float* __restrict__ v, const unsigned* __restrict__ mm, int bin,
-                          int* __restrict__ rows, unsigned* __restrict__ count) {
-    const float lo = unordered_bits(mm[0]), hi = unordered_bits(mm[1]);
-    const float scale = hi > lo ? 4096.f / (hi - lo) : 0.f;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < M; i += gridDim.x * blockDim.x) {
-        int b = (int)((v[i] - lo) * scale);
-        b = b < 0 ? 0 : (b > 4095 ? 4095 : b);
-        if (b >= bin) rows[atomicAdd(count, 1u)] = i;
-    }
-}
-
 __global__ void k_init_mm(unsigned* mm) {
     if (threadIdx.x == 0) { mm[0] = 0xffffffffu; mm[1] = 0u; mm[2] = 0u; }
 }
@@ -493,10 +483,12 @@ extern "C" int vx_select_top(int M, const float* pred, int k, int* rows, int* ns
         acc += h[bin];
         if (acc >= k) break;
     }
-    hipLaunchKernelGGL(k_compact, dim3(256), dim3(256), 0, st, M, pred, buf, bin, rows, buf + 2);
-    VX_LAUNCH_CHECK();
+    // rows in ascending order (ordered compaction; the histogram buffer is
+    // free again and holds the compaction's block counts and total)
+    int rc = compact_top_bins(M, pred, buf, bin, rows, buf + 4, st);
+    if (rc != VX_OK) return rc;
     unsigned cnt = 0;
-    VX_HIP(hipMemcpyAsync(&cnt, buf + 2, sizeof(cnt), hipMemcpyDeviceToHost, st));
+    VX_HIP(hipMemcpyAsync(&cnt, buf + 4 + 256, sizeof(cnt), hipMemcpyDeviceToHost, st));
     VX_HIP(hipStreamSynchronize(st));
     VX_HIP(hipFreeAsync(buf, st));
     *nsel_host = (int)cnt;

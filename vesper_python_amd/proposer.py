@@ -56,12 +56,31 @@ def descriptors(data: torch.Tensor, vec: torch.Tensor) -> torch.Tensor:
     return out
 
 
-def features(rot: torch.Tensor, quat: torch.Tensor, tdesc: torch.Tensor, qdesc: torch.Tensor) -> torch.Tensor:
-    """bf16 feature rows [pad64(R), IN] (zero rows past R)."""
+def features(rot: torch.Tensor, quat: torch.Tensor, tdesc: torch.Tensor, qdesc: torch.Tensor,
+             out: torch.Tensor | None = None) -> torch.Tensor:
+    """bf16 feature rows [pad64(R), IN] (zero rows past R), or written into
+    the first R rows of `out`."""
     R = int(rot.shape[0])
-    X = torch.zeros((pad64(R), IN), dtype=torch.bfloat16, device=rot.device)
+    X = torch.zeros((pad64(R), IN), dtype=torch.bfloat16, device=rot.device) if out is None else out
     nat.call("vx_mlp_features", R, IN, _p(rot), _p(quat), _p(tdesc), _p(qdesc), _p(X), _st())
     return X
+
+
+def gather_rows(src: torch.Tensor, idx: torch.Tensor, n: int, out: torch.Tensor, nout: int | None = None):
+    """out rows [0, n) = src rows idx[:n] (idx int32 on the device), zero rows
+    up to nout (vx_gather_rows)."""
+    row_bytes = src.element_size() * (src[0].numel() if src.dim() > 1 else 1)
+    nat.call("vx_gather_rows", int(n if nout is None else nout), int(n), _p(idx), _p(src), int(row_bytes), _p(out),
+             _st())
+    return out
+
+
+def zscore_into(s: torch.Tensor, n: int, out: torch.Tensor, stats: torch.Tensor, given: bool = False):
+    """out[:n] = (s[:n] - mean) / (std + 1e-30) (vx_zscore_f32); stats
+    (float64 [2] on the device) receives mean and std + 1e-30, or, given,
+    supplies them."""
+    nat.call("vx_zscore_f32", int(n), _p(s), _p(stats), int(bool(given)), _p(out), _st())
+    return out
 
 
 class RotationMLP:
@@ -109,17 +128,20 @@ class RotationMLP:
         return buf["pred"][:M]
 
     # -- training --------------------------------------------------------------
-    def grads(self, X: torch.Tensor, y: torch.Tensor, M: int):
+    def grads(self, X: torch.Tensor, y: torch.Tensor, M: int, loss: torch.Tensor | None = None):
         """Mean-squared-loss gradients of every parameter (fp32, same order as
-        self.params) and the loss, by the library's backward kernels."""
+        self.params) and the loss (added to `loss`, a device float32 [1],
+        when given), by the library's backward kernels."""
         Mp = int(X.shape[0])
         buf = self._get(Mp)
         pred = self.forward(X, M)
         g = [torch.zeros_like(p) for p in self.params]
         gW, gb, gw4, gb4 = g[0:3], g[3:6], g[6], g[7]
-        loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
+        if loss is None:
+            loss = torch.zeros(1, dtype=torch.float32, device=self.dev)
         A, D = buf["A"], buf["D"]
-        D[2][M:].zero_()      # rows past M take no part (the head writes rows < M)
+        if Mp > M:            # rows past M take no part (the head writes rows < M)
+            nat.call("vx_gather_rows", Mp - M, 0, None, None, H * 2, _p(D[2][M:]), _st())
         # output layer; D[2] = dL/dZ3 (through the ReLU of A3)
         nat.call("vx_mlp_head_bwd", M, H, _p(A[2]), H, _p(self.w4), _p(pred), _p(y), _p(gw4), _p(gb4),
                  _p(D[2]), _p(loss), _st())
@@ -134,8 +156,8 @@ class RotationMLP:
                          None, _p(A[l - 1]), H, _st())
         return g, loss
 
-    def train_step(self, X, y, M, lr=1e-3, wd=0.0):
-        g, loss = self.grads(X, y, M)
+    def train_step(self, X, y, M, lr=1e-3, wd=0.0, loss=None):
+        g, loss = self.grads(X, y, M, loss)
         self.steps += 1
         bf = self.Wb + [None] * 5
         for p, gp, a, b, pb in zip(self.params, g, self.m1, self.m2, bf):
@@ -144,7 +166,9 @@ class RotationMLP:
         return loss
 
     def fit(self, X, y, M, epochs=20, batch=2048, lr=1e-3, seed=0):
-        """Minibatch Adam over the rows of X (bf16 [Mp, IN]) / y (fp32 [M])."""
+        """Minibatch Adam over the rows of X (bf16 [>= M, IN]) / y (fp32
+        [>= M]); minibatches gathered on the device (vx_gather_rows).
+        Returns the mean minibatch loss of each epoch."""
         rng = np.random.default_rng(seed)
         bp = pad64(min(batch, M))
         xb = torch.zeros((bp, IN), dtype=torch.bfloat16, device=self.dev)
@@ -152,16 +176,16 @@ class RotationMLP:
         # per-epoch loss sums stay on the device until the end (no host sync
         # per epoch)
         tot = torch.zeros(epochs, device=self.dev)
+        nb = (M + batch - 1) // batch
         for e in range(epochs):
-            perm = torch.from_numpy(rng.permutation(M)).to(self.dev)
+            perm = torch.from_numpy(rng.permutation(M).astype(np.int32)).to(self.dev)
             for s in range(0, M, batch):
-                idx = perm[s:s + batch]
-                m = int(idx.numel())
-                xb[:m] = X.index_select(0, idx)
-                xb[m:] = 0
-                yb[:m] = y.index_select(0, idx)
-                tot[e:e + 1] += self.train_step(xb, yb, m, lr) * m
-        return [float(v) / M for v in tot.cpu().numpy()]
+                m = min(batch, M - s)
+                idx = perm[s:s + m]
+                gather_rows(X, idx, m, xb, bp)
+                gather_rows(y, idx, m, yb, bp)
+                self.train_step(xb, yb, m, lr, loss=tot[e:e + 1])
+        return [float(v) / nb for v in tot.cpu().numpy()]
 
     def state(self) -> dict:
         return {f"p{i}": p.cpu().numpy() for i, p in enumerate(self.params)}
@@ -176,12 +200,13 @@ class RotationMLP:
 def landscape(plan: engine.SearchPlan, rot: torch.Tensor) -> torch.Tensor:
     """Exhaustive float32 best scores of every row (the training labels)."""
     best, _ = plan.score_rows(rot)
-    return best[:, 0].contiguous()
+    return best.view(-1) if best.shape[1] == 1 else best[:, 0].contiguous()
 
 
 def zscore(s: torch.Tensor) -> torch.Tensor:
-    s = s.double()
-    return ((s - s.mean()) / (s.std() + 1e-30)).float()
+    """(s - mean) / (std + 1e-30) on the device."""
+    out = torch.empty_like(s)
+    return zscore_into(s, int(s.numel()), out, torch.empty(2, dtype=torch.float64, device=s.device))
 
 
 class Proposer:
@@ -193,10 +218,10 @@ class Proposer:
         self.td = descriptors(tdata, tvec)
         self.qd = descriptors(sdata, svec)
 
-    def rows_features(self, angles):
+    def rows_features(self, angles, out=None):
         rot = torch.from_numpy(rotation_rows(angles)).to(self.td.device)
         quat = torch.from_numpy(quaternions(angles)).to(self.td.device)
-        return rot, features(rot, quat, self.td, self.qd)
+        return rot, features(rot, quat, self.td, self.qd, out)
 
     def fine_tune(self, coarse_angles, epochs=30, lr=1e-3):
         """Train on this pair's own coarse landscape (every row of a coarse
@@ -215,11 +240,12 @@ class Proposer:
         rows = torch.empty(R, dtype=torch.int32, device=rot.device)
         nsel = np.zeros(1, dtype=np.int32)
         nat.call("vx_select_top", R, _p(pred), k, _p(rows), nsel.ctypes.data_as(__import__("ctypes").c_void_p), _st())
-        sel = rows[:int(nsel[0])].long().sort().values
-        best, _ = self.plan.score_rows(rot.index_select(0, sel).contiguous())
-        b = best[:, 0].contiguous()
+        ns = int(nsel[0])
+        rsel = gather_rows(rot, rows, ns, torch.empty((ns, 9), dtype=torch.float32, device=rot.device))
+        best, _ = self.plan.score_rows(rsel)
+        b = best.view(-1)
         t = engine.topn(b, topn)
-        return sel[torch.from_numpy(t.astype(np.int64)).to(sel.device)].cpu().numpy(), b, int(sel.numel())
+        return rows[:ns].cpu().numpy()[t], b, ns
 
 
 def pretrain(mlp: RotationMLP, pairs, angles, epochs=20, lr=2e-3, batch=2048, seed=0):
@@ -227,17 +253,15 @@ def pretrain(mlp: RotationMLP, pairs, angles, epochs=20, lr=2e-3, batch=2048, se
     Proposer holding the pair's plan and maps) is FFT-scored exhaustively and
     z-scored within its pair; the rows of all pairs form one training set.
     Returns the per-epoch losses."""
-    Xs, ys = [], []
-    for pr in pairs:
-        rot, X = pr.rows_features(angles)
-        R = int(rot.shape[0])
-        Xs.append(X[:R])
-        ys.append(zscore(landscape(pr.plan, rot)))
-    X = torch.cat(Xs).contiguous()
-    y = torch.cat(ys).contiguous()
-    M = int(y.numel())
-    Xp = torch.zeros((pad64(M), IN), dtype=X.dtype, device=X.device)
-    Xp[:M] = X
+    R = len(angles)
+    M = R * len(pairs)
+    dev = mlp.dev
+    Xp = torch.zeros((pad64(M), IN), dtype=torch.bfloat16, device=dev)
+    y = torch.empty(M, dtype=torch.float32, device=dev)
+    stats = torch.empty(2, dtype=torch.float64, device=dev)
+    for k, pr in enumerate(pairs):
+        rot, _ = pr.rows_features(angles, out=Xp[k * R:])
+        zscore_into(landscape(pr.plan, rot), R, y[k * R:], stats)
     return mlp.fit(Xp, y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
 
 
@@ -308,63 +332,75 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
     if policy not in POLICIES:
         raise ValueError(f"unknown policy {policy!r}")
     dev = pr.td.device
-    crot, cX = pr.rows_features(coarse_angles)
-    cs = landscape(pr.plan, crot).double()
-    cmean, cstd = cs.mean(), cs.std() + 1e-30
-    cy = ((cs - cmean) / cstd).float()
+    Rc = len(coarse_angles)
+    R = len(angles)
+    # training set: the coarse rows, then every fine row scored (capacity)
+    trainX = torch.zeros((pad64(Rc + R), IN), dtype=torch.bfloat16, device=dev)
+    trainy = torch.empty(Rc + R, dtype=torch.float32, device=dev)
+    cstats = torch.empty(2, dtype=torch.float64, device=dev)
+    crot, _ = pr.rows_features(coarse_angles, out=trainX)
+    cs = landscape(pr.plan, crot)
+    zscore_into(cs, Rc, trainy, cstats)
+    ntrain = Rc
     rot, X = pr.rows_features(angles)
-    R = int(rot.shape[0])
-    scored = torch.zeros(R, dtype=torch.bool, device=dev)
-    best_all = torch.full((R,), -float("inf"), dtype=torch.float32, device=dev)
-    trainX, trainy = [cX[:int(crot.shape[0])]], [cy]
+    scored = torch.zeros(R, dtype=torch.uint8, device=dev)
+    best_all = torch.empty(R, dtype=torch.float32, device=dev)
+    nat.call("vx_fill_f32", R, -float("inf"), _p(best_all), _st())
+    rows = torch.empty(R, dtype=torch.int32, device=dev)
+    sel = torch.empty(R, dtype=torch.int32, device=dev)
+    rsel = torch.empty((R, 9), dtype=torch.float32, device=dev)
+    pred_buf = torch.empty(R, dtype=torch.float32, device=dev)
+    nscored = 0
     share = max(topn, int(math.ceil(frac * R / rounds)))
     static_pred = None
     if policy == "coarse":
         near, dist = nearest_coarse_rows(angles, coarse_step)
-        base = cy.double().cpu().numpy()[near]
+        base = trainy[:Rc].cpu().numpy().astype(np.float64)[near]
         static_pred = torch.from_numpy((base - 1e-6 * dist).astype(np.float32)).to(dev)
     elif policy == "random":
         static_pred = torch.from_numpy(np.random.default_rng(seed).random(R).astype(np.float32)).to(dev)
+    cnt = np.zeros(1, dtype=np.int32)
+    cnt_p = cnt.ctypes.data_as(__import__("ctypes").c_void_p)
+
+    def score_new(n):
+        """FFT-score the n rows in sel; record them and their scores."""
+        gather_rows(rot, sel, n, rsel)
+        b, _ = pr.plan.score_rows(rsel[:n])
+        nat.call("vx_scatter_scores", n, _p(sel), _p(b), int(b.shape[1]), _p(best_all), _p(scored), _st())
+        return b
+
     for r in range(rounds):
         if static_pred is None:
-            Xt = torch.cat(trainX)
-            yt = torch.cat(trainy)
-            pr.mlp.fit(Xt, yt, int(yt.numel()), epochs=epochs if r == 0 else max(5, epochs // 4), batch=512,
-                       lr=lr, seed=seed + r)
-            pred = pr.mlp.forward(X, R).clone()
+            pr.mlp.fit(trainX, trainy, ntrain, epochs=epochs if r == 0 else max(5, epochs // 4), batch=512, lr=lr,
+                       seed=seed + r)
+            pred = pr.mlp.forward(X, R)
         else:
-            pred = static_pred.clone()
-        if bool(scored.any()):
-            # rows already scored drop just below the lowest unscored prediction
-            # (finite, so the selection histogram keeps its resolution)
-            low = pred[~scored].min()
-            pred[scored] = low - 1e-3 * low.abs() - 1e-3
-        rows = torch.empty(R, dtype=torch.int32, device=dev)
-        nsel = np.zeros(1, dtype=np.int32)
-        nat.call("vx_select_top", R, _p(pred), share, _p(rows), nsel.ctypes.data_as(__import__("ctypes").c_void_p),
-                 _st())
-        sel = rows[:int(nsel[0])].long()
-        sel = sel[~scored[sel]].sort().values
-        if sel.numel() == 0:
+            pred = pred_buf.copy_(static_pred)
+        # rows already scored drop just below the lowest unscored prediction
+        nat.call("vx_mask_scored", R, _p(pred), _p(scored), _st())
+        nat.call("vx_select_top", R, _p(pred), share, _p(rows), cnt_p, _st())
+        nat.call("vx_filter_unscored", int(cnt[0]), _p(rows), _p(scored), _p(sel), cnt_p, _st())
+        n = int(cnt[0])
+        if n == 0:
             break
-        b, _ = pr.plan.score_rows(rot.index_select(0, sel).contiguous())
-        best_all[sel] = b[:, 0]
-        scored[sel] = True
-        trainX.append(X.index_select(0, sel))
+        b = score_new(n)
+        nscored += n
+        gather_rows(X, sel, n, trainX[ntrain:])
         # labels on the coarse landscape's scale
-        trainy.append(((b[:, 0].double() - cmean) / cstd).float())
+        zscore_into(b, n, trainy[ntrain:], cstats, given=True)
+        ntrain += n
     shape = grid_shape(angles) if sweeps else None
     for _ in range(sweeps if shape else 0):
         top = engine.topn(best_all, topn)
-        nb = torch.from_numpy(neighbour_rows(top, *shape)).to(dev)
-        nb = nb[~scored[nb]]
-        if nb.numel() == 0:
+        nb = torch.from_numpy(neighbour_rows(top, *shape).astype(np.int32)).to(dev)
+        nat.call("vx_filter_unscored", int(nb.numel()), _p(nb), _p(scored), _p(sel), cnt_p, _st())
+        n = int(cnt[0])
+        if n == 0:
             break
-        b, _ = pr.plan.score_rows(rot.index_select(0, nb).contiguous())
-        best_all[nb] = b[:, 0]
-        scored[nb] = True
+        score_new(n)
+        nscored += n
     top = engine.topn(best_all, topn)
-    return top, int(scored.sum().item()), int(crot.shape[0])
+    return top, nscored, Rc
 
 
 def recall(a, b) -> float:
