@@ -123,11 +123,6 @@ int vx_plan_destroy(vx_plan* plan);
 #define VX_TUNE_RADIX_ORDER 9 /* order of the FFT stages: 0 (default) = radix 8, 4, 2 then the odd
                                  primes ascending; 1 = reversed; 2 = odd primes ascending first;
                                  3 = odd primes descending first.  Results equal to fp32 rounding. */
-#define VX_TUNE_SLAB_PERSIST 10 /* 1: the forward and inverse slab passes run as about one workgroup
-                                   per CU, each walking slabs with the next slab's loads in flight
-                                   during the current one's transforms (compile-time plans,
-                                   1024-thread slabs); 0 = one workgroup per slab (default).
-                                   Identical results. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,
