@@ -263,6 +263,46 @@ def measure_prob(n1, n2, rot, batch, steps):
             "schedule": tuned["schedule"], "stage_ms_per_launch": {k: round(v / 8, 4) for k, v in stage.items()}}
 
 
+def measure_rescore(plan, n1, n2, angles, best):
+    """The FP64 re-check on the benchmark pair: the rows near the top-10
+    cutoff of the exhaustive float32 scores and the 3 x 3 x 3 refinement rows
+    of the top 10 (at most 270) re-scored by vx_rescore_f64; GPU time (HIP
+    events), candidates per row, the largest FP32/FP64 relative difference,
+    and how many of the top 10 change rank or translation."""
+    import torch
+    from vesper_python_amd import engine
+    from vesper_python_amd import VESPER_1_prob_new as V
+    from vesper_python_amd.geometry import refine_rows, rotation_rows
+    b32 = best[:, 0].double().cpu().numpy()
+    top = engine.topn(best[:, 0].contiguous(), 10)
+    near = V.recheck_rows(b32, top)
+    ref_rows = np.concatenate([refine_rows(angles[i]) for i in top], axis=0)
+    rows = np.concatenate([angles[near], ref_rows], axis=0)
+    rot = torch.from_numpy(rotation_rows(rows)).cuda()
+    target, search = n1._dev, n2._dev
+    plan.rescore_f64(rot[:8], target, search, n1.ave)             # warm-up (workspace, kernels)
+    b, t = plan.score_rows(rot)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    sc, tr, nc = plan.rescore_f64(rot, target, search, n1.ave)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    s64 = sc[:, 0].cpu().numpy()
+    s32 = b[:, 0].double().cpu().numpy()
+    t64, t32 = tr[:, 0].cpu().numpy(), t[:, 0].cpu().numpy()
+    k = len(near)
+    order64 = sorted(range(k), key=lambda i: (-s64[i], int(near[i])))[:10]
+    return {"rows": int(len(rows)), "near_cutoff_rows": int(k), "refinement_rows": int(len(ref_rows)),
+            "gpu_ms": round(ms, 3), "ms_per_row": round(ms / len(rows), 4),
+            "mean_candidates": round(float(nc.double().mean().item()), 2), "max_candidates": int(nc.max().item()),
+            "max_rel_fp32_vs_fp64": float(np.max(np.abs(s32 - s64) / np.abs(s64))),
+            "translations_changed": int(np.sum(np.any(t32 != t64, axis=1))),
+            "top10_rank_changes": int(sum(int(near[a]) != int(b_) for a, b_ in zip(order64, top)))}
+
+
 def measure_library_fft(n1, n2, rot, batch, plan):
     """The same correlation done with the vendor FFT library (rocFFT through
     torch.fft) on already-rotated query channels: forward r2c of the three
@@ -397,7 +437,9 @@ def measure_pairs48(npairs=48, ang=10.0, s=1.5, g=16.0, seed0=4000, workers=16):
     rows = len(calc_angle_comb(ang)) * npairs
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    out = batch.search_pairs(pairs, ang=ang, use_graph=True)
+    out = batch.search_pairs(pairs, ang=ang, use_graph=True,
+                             progress=lambda i, n: print(f"[pairs48] searched pair {i + 1} N={n}", file=sys.stderr,
+                                                         flush=True))
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     edges = [int(p[0].xdim) for p in pairs]
@@ -783,6 +825,10 @@ def main():
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
 
+    rescore = None
+    if not args.rows:
+        rescore = measure_rescore(plan, n1, n2, angles, best)
+
     libfft = None
     if not args.no_libfft and not args.rows:
         libfft = measure_library_fft(n1, n2, rot, args.batch, plan)
@@ -827,6 +873,7 @@ def main():
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
            "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m,
+           "fp64_recheck": rescore,
            "cli_end_to_end": cli_m}
     if cpu:
         cpu["parity_vs_gpu"] = baseline_parity(cpu, best, trans)

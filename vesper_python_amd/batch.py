@@ -68,10 +68,11 @@ def _mode_scale(target, mode):
     return 1.0
 
 
-def search_pairs(pairs, ang=10.0, mode="VecProduct", topn=10, batch=None, use_graph=True):
+def search_pairs(pairs, ang=10.0, mode="VecProduct", topn=10, batch=None, use_graph=True, progress=None):
     """Exhaustive search of every (target, search) pair of resampled maps
     (fastVEC outputs).  Returns per pair a dict(n, scores (1/N^3-scaled,
-    float64 [R]), trans [R, 3], top (row ids), angles)."""
+    float64 [R]), trans [R, 3], top (row ids), angles).  `progress`, if
+    given, is called with (pair index, grid edge) after each pair."""
     angles = calc_angle_comb(ang)
     rot_host = rotation_rows(angles)
     dev = engine.require_gpu()
@@ -115,5 +116,7 @@ def search_pairs(pairs, ang=10.0, mode="VecProduct", topn=10, batch=None, use_gr
                 top = engine.topn(best[:, 0].contiguous(), topn)
             b = best[:, 0].cpu().numpy().astype(np.float64) * _mode_scale(t, mode) / n ** 3
             out[i] = dict(n=n, scores=b, trans=trans[:, 0].cpu().numpy(), top=np.asarray(top), angles=angles)
+            if progress is not None:
+                progress(i, n)
         plan.close()
     return out
