@@ -55,7 +55,7 @@ def _stage_bytes(n: int, w: int, b: int) -> dict:
             "inv_yz": b * nh * 8}
 
 
-PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 / fp16 MFMA
 
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "xcorr_pair200_pmc.md")
 
@@ -98,6 +98,7 @@ def prepare(workload: str):
                                             w["s"], w["g"])
         n1 = V.fastVEC(m1, n1, w["g"])
         n2 = V.fastVEC(m2, n2, w["g"])
+    w["src_maps"] = (m1, m2)
     return w, n1, n2
 
 
@@ -301,6 +302,71 @@ def measure_rescore(plan, n1, n2, angles, best):
             "max_rel_fp32_vs_fp64": float(np.max(np.abs(s32 - s64) / np.abs(s64))),
             "translations_changed": int(np.sum(np.any(t32 != t64, axis=1))),
             "top10_rank_changes": int(sum(int(near[a]) != int(b_) for a, b_ in zip(order64, top)))}
+
+
+def _event_ms(fn, reps):
+    """Mean milliseconds of fn() between HIP events on the current stream,
+    after one untimed call."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(reps):
+        fn()
+    e1.record(st)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def measure_aux(w, n1, n2, best, reps=10):
+    """HBM rooflines of the kernels beside the search on the benchmark pair,
+    each call timed with HIP events on its launch stream (mean of `reps`
+    calls after a warm-up), against its algorithmic bytes:
+      fastVEC of the query map (k_fv_z reads the source map and writes the z
+        sums, 2 float64 per (x, y, dest z); k_fv_y reads them and writes 3
+        float64 per (x, dest y, dest z); k_fv_x reads those, one source tap
+        per voxel, and writes data float64 + vec 3 x float32),
+      get_score at one translation (k_stats1 + k_stats2 read the query
+        density twice; k_score reads both maps' density float64 and vectors
+        3 x float32),
+      the top-10 selection over every row's score (vx_topn_ws: the scores
+        once; a few launches of sorting in LDS, latency-bound).
+    The per-kernel bytes divide the per-kernel durations of a kernel trace."""
+    import torch
+    from vesper_python_amd import engine
+    m1, m2 = w["src_maps"]
+    n = int(n2.xdim)
+    n3 = n ** 3
+    src = getattr(m2, "_dev_data", None)
+    if src is None:
+        src = engine.to_dev(np.asarray(m2.data, dtype=np.float32), torch.float32)
+    src = src.float().contiguous()
+    sx, sy, sz = (int(v) for v in src.shape)
+    fv_bytes = {"k_fv_z": sx * sy * sz * 4 + sx * sy * n * 16,
+                "k_fv_y": sx * sy * n * 16 + 3 * sx * n * n * 8,
+                "k_fv_x": 3 * sx * n * n * 8 + n3 * 4 + n3 * (8 + 12)}
+    fv_ms = _event_ms(lambda: engine.fastvec(src, m2.xwidth, m2.orig, n, n2.xwidth, n2.orig, w["g"]), reps)
+    tdata, tvec = n1._dev
+    sdata, svec = n2._dev
+    gs_bytes = {"k_stats1+k_stats2": 2 * n3 * 8, "k_score": n3 * (8 + 12 + 8 + 12)}
+    gs_ms = _event_ms(lambda: engine.get_score(tdata, tvec, n1.ave, sdata, svec, (3, -2, 5)), reps)
+    R = int(best.shape[0])
+    scores = best[:, 0].contiguous()
+    ws = engine.topn_workspace(R, 10)
+    rows = torch.empty(10, dtype=torch.int32, device=scores.device)
+    tn_ms = _event_ms(lambda: engine.topn_into(scores, R, 10, rows, ws), reps)
+    tn_bytes = R * 4 + 10 * 4
+
+    def line(ms, by, bound="hbm"):
+        gbs = sum(by.values()) / (ms / 1e3) / 1e9
+        return {"ms": round(ms, 4), "bytes": int(sum(by.values())), "by_kernel_bytes": by, "achieved_GBps": round(gbs, 1),
+                "frac": round(gbs / PEAK_HBM_GBS, 4), "bound": bound}
+    return {"fastvec": dict(line(fv_ms, fv_bytes), source_shape=[sx, sy, sz], n=n),
+            "get_score": line(gs_ms, gs_bytes),
+            "topn": dict(line(tn_ms, {"vx_topn_ws": tn_bytes}, "latency"), rows=R)}
 
 
 def measure_library_fft(n1, n2, rot, batch, plan):
@@ -711,6 +777,8 @@ def main():
                     help="only the 48-pair uneven workload at -s 1.5 -A 10; prints its own JSON line")
     ap.add_argument("--ablation", action="store_true",
                     help="only the learned-proposer ablation (held-out pairs, -A 5 judge); prints its own JSON line")
+    ap.add_argument("--aux", action="store_true",
+                    help="only the rooflines of fastVEC, get_score and top-N on the workload; prints its own JSON line")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
                     'e.g. \'{"xpass_variant": 2, "xpass_tile": 64}\' (profiling a specific kernel)')
     args = ap.parse_args()
@@ -745,6 +813,11 @@ def main():
         angles = angles[:args.rows]
     R = len(angles)
     dev = torch.device("cuda", 0)
+    if args.aux:
+        # top-N timing needs one score per row, not the search's values
+        scores = torch.rand((R, 1), generator=torch.Generator(device=dev).manual_seed(0), device=dev)
+        print(json.dumps({"aux_rooflines": measure_aux(w, n1, n2, scores)}), flush=True)
+        return
     tdata, tvec = n1._dev
     sdata, svec = n2._dev
     plan = engine.SearchPlan(n, nat.VX_MODE_VEC, batch=args.batch)
@@ -821,13 +894,30 @@ def main():
         tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
         roofline["mfma"] = {"kernel": "xcorr", "flops_per_launch": flops, "achieved": round(tflops, 1),
                             "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
+    if isinstance(sched, dict) and sched.get("xpass_variant") == 4:
+        # radix-split MFMA x pass: per 16-column tile and rotation, 4 waves x
+        # 3 split products x N2 column blocks x (channels x forward K-steps +
+        # inverse K-steps) v_mfma_f32_16x16x32_f16 of 16,384 flops each
+        from vesper_python_amd.autotune import mr_factor
+        n1, n2 = mr_factor(n)
+        lo_, w_ = info["box_lo"], info["box_width"]
+        nwin = (lo_ + w_ - 1) // n2 - lo_ // n2 + 1
+        ksf, ksi = (1 if nwin <= 16 else 2), (1 if n1 <= 16 else 2)
+        tiles = -(-(n * (n // 2 + 1)) // 16)
+        flops = args.batch * tiles * 4 * 3 * n2 * (3 * ksf + ksi) * 16384
+        tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
+        roofline["mfma"] = {"kernel": "xcorr (radix split, N = %d x %d)" % (n1, n2), "flops_per_launch": flops,
+                            "achieved": round(tflops, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
+                            "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
     if dom == "xcorr":
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
 
     rescore = None
+    aux = None
     if not args.rows:
         rescore = measure_rescore(plan, n1, n2, angles, best)
+        aux = measure_aux(w, n1, n2, best)
 
     libfft = None
     if not args.no_libfft and not args.rows:
@@ -873,7 +963,7 @@ def main():
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
            "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m,
-           "fp64_recheck": rescore,
+           "fp64_recheck": rescore, "aux_rooflines": aux,
            "cli_end_to_end": cli_m}
     if cpu:
         cpu["parity_vs_gpu"] = baseline_parity(cpu, best, trans)

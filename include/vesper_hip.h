@@ -97,9 +97,13 @@ int vx_plan_destroy(vx_plan* plan);
  * 0 = FFT x pass accumulating channels in registers, 1 = in LDS, 3 = in
  * registers with the next channel prefetched (tiles of N x tile <= 2048; 0,
  * 1 and 3 give identical results), 2 = split-bf16 MFMA DFT x pass (one
- * correlation group, N <= 192; scores within 1e-4 relative of the FFT pass).
- * VX_TUNE_XPASS_TILE: columns per x-pass tile (0 = default; 32 or 64 for
- * variant 2).  VX_TUNE_SLAB_THREADS: threads per z/y slab workgroup (0 =
+ * correlation group, N <= 192; scores within 1e-4 relative of the FFT pass),
+ * 4 = radix-split x pass: N = N1 x N2 (N1 <= 32, N2 in 2, 3, 4, 5, 7, 8), the
+ * length-N1 DFTs on split-f16 MFMA with per-tile power-of-two scaling, the
+ * radix-N2 stage in fp32 registers (one correlation group; scores within
+ * 1e-5 relative of the FFT pass).  vx_rescore_f64 always takes its
+ * candidates from the FFT x pass.  VX_TUNE_XPASS_TILE: columns per x-pass
+ * tile (0 = default; 32 or 64 for variant 2; variant 4 uses 16).  VX_TUNE_SLAB_THREADS: threads per z/y slab workgroup (0 =
  * default = 1024 where the slab fits that shape, else 512; identical results).
  * Grids whose (y, z) slab exceeds LDS always use the pencil passes.
  */
@@ -185,7 +189,8 @@ int vx_score_rows(vx_plan* plan, const float* rot_dev, int nrows, float* best_de
  *                                  max_cand of them were summed and the row
  *                                  needs a smaller window or a larger max_cand
  * The first call, and any larger one, grows a plan-owned workspace (batch x
- * groups x n^3 float32 maps plus candidate lists).  Not for the MFMA x pass.
+ * groups x n^3 float32 maps plus candidate lists).  With an MFMA x pass
+ * selected, the float32 maps of the candidates come from the FFT x pass.
  */
 int vx_rescore_f64(vx_plan* plan, const double* tdata_dev, const float* tvec_dev, const float* tprob_dev,
                    double t_ave, const double* sdata_dev, const float* svec_dev, const float* sprob_dev,

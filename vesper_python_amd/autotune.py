@@ -6,10 +6,10 @@ pencil passes, the order of the FFT's radix stages and the number of batches
 in flight on a sample of the
 actual rotation rows with the library's per-stage HIP-event profiler
 (vx_score_rows_profile), keeps the fastest, and caches it per
-(device, N, mode, batch) in a JSON file.  The FFT schedules (variants 0, 1)
-compute identical results; the split-bf16 MFMA DFT x pass (variant 2) is
-accepted only if its best scores on the sample agree with the FFT schedule's
-within 1e-4 of the largest score.
+(device, N, mode, batch) in a JSON file.  The FFT schedules (variants 0, 1,
+3) compute identical results; the MFMA x passes (variant 2, split-bf16 DFT;
+variant 4, split-f16 radix split) are accepted only if their best scores on
+the sample agree with the FFT schedule's within 1e-4 of the largest score.
 """
 from __future__ import annotations
 
@@ -39,7 +39,22 @@ def candidates(n: int, groups: int = 1):
     if groups == 1 and n <= 192:
         for t in (32, 64):
             out.append(dict(xpass_variant=2, xpass_tile=t))
+    if groups == 1 and mr_factor(n) is not None:
+        out.append(dict(xpass_variant=4, xpass_tile=0))
     return out
+
+
+MFMA_VARIANTS = (2, 4)
+
+
+def mr_factor(n: int):
+    """(N1, N2) of the radix-split MFMA x pass for grid edge n: N2 the
+    smallest of 2, 3, 4, 5, 7, 8 dividing n with N1 = n / N2 <= 32, else None
+    (the library's choice)."""
+    for r in (2, 3, 4, 5, 7, 8):
+        if n % r == 0 and 2 <= n // r <= 32:
+            return n // r, r
+    return None
 
 
 def radix_orders(n: int) -> dict:
@@ -113,7 +128,7 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
                 print("[tune] skip", cand, e, file=sys.stderr)
             continue
         got = got.double().cpu().numpy()
-        if cand["xpass_variant"] != 2:
+        if cand["xpass_variant"] not in MFMA_VARIANTS:
             exact = got if exact is None else exact
         elif exact is None or np.max(np.abs(got - exact)) > APPROX_RTOL * max(np.max(np.abs(exact)), 1e-30):
             if verbose:

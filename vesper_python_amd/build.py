@@ -19,7 +19,7 @@ INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libvesper_hip.so")
 SOURCES = ["vx_search.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip",
-           "vx_noalloc.hip", "vx_rescore.hip", "vx_propose.hip"]
+           "vx_noalloc.hip", "vx_rescore.hip", "vx_propose.hip", "vx_xmr.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VX_OFFLOAD_ARCH", "gfx950")
@@ -33,6 +33,11 @@ CFLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC, "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
 ]
+
+
+# Per-source additions: the radix-split MFMA x pass keeps its accumulators in
+# ordinary VGPRs (no AGPR copies), which leaves it 3 waves per SIMD at N = 175.
+EXTRA_FLAGS = {"vx_xmr.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _stale(src: str, obj: str) -> bool:
@@ -49,7 +54,7 @@ def _compile(src: str, incremental: bool = False) -> str:
     obj = os.path.join(OBJDIR, os.path.splitext(src)[0] + ".o")
     if incremental and not _stale(src, obj):
         return obj
-    cmd = [HIPCC, *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *EXTRA_FLAGS.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

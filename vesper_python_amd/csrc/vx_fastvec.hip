@@ -27,7 +27,8 @@ struct AxisGeom {
     int* len;       // [3][n] window length (0 if empty)
     int* inb;       // [3][n] 0 <= pos < dim
     int* ipos;      // [3][n] int(pos)
-    double* w;      // [3][n][W]
+    double* w;      // [3][W][n]: tap j of destination voxel d at w[(a W + j) n + d], so that
+                    // neighbouring destinations (the lanes of k_fv_z) read neighbouring weights
 };
 
 // Per-axis geometry (VESPER_1_prob_new.py:196, :214-232) and weights.
@@ -61,7 +62,7 @@ __global__ void k_fv_axis(AxisGeom g, int n, int W, int sx, int sy, int sz, doub
             const double r = __dsub_rn((double)(stp + j), pos);
             w = exp(__dmul_rn(__dmul_rn(-1.5, __dmul_rn(r, r)), fsiv));
         }
-        g.w[(size_t)i * W + j] = w;
+        g.w[((size_t)a * W + j) * n + d] = w;
     }
 }
 
@@ -74,11 +75,11 @@ __global__ void k_fv_z(const float* __restrict__ src, AxisGeom g, int n, int W, 
     const int dz = (int)(idx % n);
     const size_t line = idx / n;   // xp * sy + yp
     const int st = g.stp[2 * n + dz], len = g.len[2 * n + dz];
-    const double* w = g.w + (size_t)(2 * n + dz) * W;
+    const double* w = g.w + (size_t)2 * W * n + dz;
     const float* s = src + line * sz;
     double a0 = 0.0, a1 = 0.0;
     for (int j = 0; j < len; ++j) {
-        const double v = w[j] * (double)s[st + j];
+        const double v = w[(size_t)j * n] * (double)s[st + j];
         a0 += v;
         a1 += v * (double)(st + j);
     }
@@ -95,14 +96,15 @@ __global__ void k_fv_y(const double2* __restrict__ A, AxisGeom g, int n, int W, 
     const int dy = (int)((idx / n) % n);
     const size_t xp = idx / ((size_t)n * n);
     const int st = g.stp[n + dy], len = g.len[n + dy];
-    const double* w = g.w + (size_t)(n + dy) * W;
+    const double* w = g.w + (size_t)W * n + dy;
     double b0 = 0.0, b1 = 0.0, b2 = 0.0;
     for (int j = 0; j < len; ++j) {
         const double2 a = A[(xp * sy + (st + j)) * n + dz];
-        const double v = w[j] * a.x;
+        const double wj = w[(size_t)j * n];
+        const double v = wj * a.x;
         b0 += v;
         b1 += v * (double)(st + j);
-        b2 += w[j] * a.y;
+        b2 += wj * a.y;
     }
     const size_t plane = (size_t)sx * n * n;
     B[idx] = b0;
@@ -129,18 +131,19 @@ __global__ void k_fv_x(const double* __restrict__ B, const float* __restrict__ s
     }
     if (active) {
         const int st = g.stp[dx], len = g.len[dx];
-        const double* w = g.w + (size_t)dx * W;
+        const double* w = g.w + dx;
         const size_t plane = (size_t)sx * n * n;
         const size_t off = (size_t)dy * n + dz;
         double D = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
         for (int j = 0; j < len; ++j) {
             const size_t bi = (size_t)(st + j) * n * n + off;
             const double b0 = B[bi];
-            const double v = w[j] * b0;
+            const double wj = w[(size_t)j * n];
+            const double v = wj * b0;
             D += v;
             mx += v * (double)(st + j);
-            my += w[j] * B[plane + bi];
-            mz += w[j] * B[2 * plane + bi];
+            my += wj * B[plane + bi];
+            mz += wj * B[2 * plane + bi];
         }
         outd = D;
         if (D != 0.0) {

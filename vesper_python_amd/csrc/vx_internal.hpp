@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 #include <stdint.h>
+#include <vector>
 
 #include "vx_fft.hpp"
 
@@ -13,6 +14,20 @@ namespace vx {
 constexpr size_t kMaxLds = 160 * 1024 - 512;
 
 struct RotF { float m[9]; };
+
+// Workgroup -> (column tile, rotation) of the x pass.  Workgroups are handed
+// to the 8 XCDs round-robin by linear id, so XCD k = id % 8 is given the
+// contiguous tile range [k tpx, (k + 1) tpx) and, within it, every rotation
+// of a tile back to back: neighbouring tiles share 128-byte lines of the
+// spectra and the target tile is reused by all rotations from that XCD's L2.
+__device__ __forceinline__ bool xcd_tile(int id, int nb, int ntiles, int* tile, int* b) {
+    const int tpx = (ntiles + 7) >> 3;
+    const int j = id >> 3;
+    const int tl = j / nb;
+    *b = j - tl * nb;
+    *tile = (id & 7) * tpx + tl;
+    return tl < tpx && *tile < ntiles;
+}
 
 // Gather kinds (what a destination voxel collects from the source map).
 enum { KIND_VEC = 0, KIND_SCALAR = 1, KIND_PROB = 2, KIND_DATA64 = 3 };
@@ -50,6 +65,13 @@ int rs_launch_direct(const RsInputs& in, int n, int lo, int w, int fma, int gout
 int rs_launch_select(int nrows, int gout, int n, int max_cand, int prob, const int* ncand, const int* cand,
                      const double* partial, const RsStats& st, double* score, int32_t* trans, hipStream_t s);
 size_t rs_partial_doubles(int nrows, int gout, int max_cand);
+
+// x pass with the length-N1 DFTs of N = N1 N2 on split-f16 MFMA (vx_xmr.hip);
+// wtab holds W_N^j, j < N.
+bool xcorr_mr_factor(int n, int* n1, int* n2);
+int xcorr_mr_tables(int n, int lo, int w, std::vector<uint32_t>& out);
+int launch_xcorr_mr(const float2* spec, const float2* tgt, float2* out, const float2* wtab, const void* atab, int n,
+                    int nb, int C, int lo, int w, hipStream_t st);
 
 // Ordered compaction of the rows whose prediction falls in histogram bins >=
 // bin (vx_propose.hip); scratch holds 257 unsigned.

@@ -47,16 +47,34 @@ __device__ __forceinline__ void block_sum_store(double (&v)[K], double* __restri
     }
 }
 
+constexpr int kRedBlocks = 512;
+
+// One block of kRedBlocks threads adds the partials of kRedBlocks blocks:
+// thread b holds block b's K sums, then a fixed tree (xor shuffles within
+// each wave, then the waves' totals in wave order), so the order of the
+// additions is the same on every call.
 template <int K>
-__global__ void k_reduce_partials(const double* __restrict__ partial, int nblocks, double* __restrict__ out) {
-    if (threadIdx.x < K) {
+__global__ void __launch_bounds__(kRedBlocks) k_reduce_partials(const double* __restrict__ partial, int nblocks,
+                                                                double* __restrict__ out) {
+    __shared__ double s[K][kRedBlocks / 64];
+    const int b = threadIdx.x;
+    double v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = b < nblocks ? partial[(size_t)b * K + k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, 64);
+    const int lane = b & 63, wid = b >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[k][wid] = v[k];
+    __syncthreads();
+    if (b < K) {
         double t = 0.0;
-        for (int b = 0; b < nblocks; ++b) t += partial[(size_t)b * K + threadIdx.x];
-        out[threadIdx.x] = t;
+        for (int w = 0; w < kRedBlocks / 64; ++w) t += s[b][w];
+        out[b] = t;
     }
 }
-
-constexpr int kRedBlocks = 512;
 
 // pass 1: {sum all, count != 0, count > 0, sum > 0, sum^2 > 0}
 __global__ void __launch_bounds__(256) k_stats1(const double* __restrict__ d, size_t count, double* __restrict__ partial) {
@@ -251,11 +269,11 @@ static int stats_device(const double* d, size_t count, double* totals /*dev: 5 +
                         hipStream_t st) {
     hipLaunchKernelGGL(k_stats1, dim3(kRedBlocks), dim3(256), 0, st, d, count, partial);
     VX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials<5>, dim3(1), dim3(64), 0, st, partial, kRedBlocks, totals);
+    hipLaunchKernelGGL(k_reduce_partials<5>, dim3(1), dim3(kRedBlocks), 0, st, partial, kRedBlocks, totals);
     VX_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_stats2, dim3(kRedBlocks), dim3(256), 0, st, d, count, totals, partial);
     VX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(64), 0, st, partial, kRedBlocks, totals + 5);
+    hipLaunchKernelGGL(k_reduce_partials<1>, dim3(1), dim3(kRedBlocks), 0, st, partial, kRedBlocks, totals + 5);
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
@@ -310,7 +328,7 @@ extern "C" int vx_get_score(int n, const double* td, const float* tv, double t_a
     hipLaunchKernelGGL(k_score, dim3(kRedBlocks), dim3(256), 0, st, n, td, tv, t_ave, sd, sv, totals, t[0], t[1],
                        t[2], partial);
     VX_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_reduce_partials<6>, dim3(1), dim3(64), 0, st, partial, kRedBlocks, totals + 8);
+    hipLaunchKernelGGL(k_reduce_partials<6>, dim3(1), dim3(kRedBlocks), 0, st, partial, kRedBlocks, totals + 8);
     VX_LAUNCH_CHECK();
     double h[14];
     VX_HIP(hipMemcpyAsync(h, totals, sizeof(h), hipMemcpyDeviceToHost, st));

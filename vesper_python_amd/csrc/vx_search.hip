@@ -535,19 +535,6 @@ k_xcorr(const float2* __restrict__ spec, const float2* __restrict__ tgt, float2*
 // laid out so that the B workgroups of one tile share an XCD (id mod 8) and
 // are dispatched together: the target tile is read from HBM once per batch
 // and from that XCD's L2 by the other rotations.
-// Workgroup -> (column tile, rotation) of the x pass.  Workgroups are handed
-// to the 8 XCDs round-robin by linear id, so XCD k = id % 8 is given the
-// contiguous tile range [k tpx, (k + 1) tpx) and, within it, every rotation
-// of a tile back to back: neighbouring tiles share 128-byte lines of the
-// spectra and the target tile is reused by all rotations from that XCD's L2.
-__device__ __forceinline__ bool xcd_tile(int id, int nb, int ntiles, int* tile, int* b) {
-    const int tpx = (ntiles + 7) >> 3;
-    const int j = id >> 3;
-    const int tl = j / nb;
-    *b = j - tl * nb;
-    *tile = (id & 7) * tpx + tl;
-    return tl < tpx && *tile < ntiles;
-}
 
 template <int MAXE, int G>
 __global__ void __launch_bounds__(256)
@@ -1993,7 +1980,8 @@ struct vx_plan {
     bool q_dirty;     // q / qd hold non-zero values outside the support box
     int src_lo, src_w; // box of the compact gather source (svec / sscal / sprob)
     int lo, w;        // support box [lo, lo + w)^3 of the rotated query
-    int xvariant;     // x-pass kernel: 0 register FFT, 1 LDS FFT, 2 MFMA DFT, 3 register-prefetch FFT
+    int xvariant;     // x-pass kernel: 0 register FFT, 1 LDS FFT, 2 MFMA DFT, 3 register-prefetch FFT,
+                      // 4 MFMA radix split (N = N1 N2, length-N1 DFTs on split-f16 MFMA)
     int xT;           // x-pass tile width (columns), 0 = default
     int pencil;       // 1: separate z and y passes (slab does not fit LDS, or forced)
     int pin_w;        // > 0: support box fixed to [pin_lo, pin_lo + pin_w) (graph replay across pairs)
@@ -2012,6 +2000,9 @@ struct vx_plan {
     unsigned short* dft;     // MFMA x pass: split bf16 F (hi, lo) then G (hi, lo), fragment order
     size_t dft_elems[2];     // elements of one F matrix, one G matrix
     int dft_lo, dft_w, dft_kp; // box the F matrices were built for (dft_w = 0: none)
+    float2* mr_w;            // variant 4: W_N^j = exp(-2 pi i j / N), j < N
+    uint32_t* mr_a;          // variant 4: A operands of the support box (xcorr_mr_tables)
+    int mr_lo, mr_bw;        // box they were built for (mr_bw = 0: none)
     double alpha, vstd, vave, pstd, pave;
     size_t bytes;
     // FP64 re-scoring workspace (vx_rescore_f64), grown on demand
@@ -2029,7 +2020,7 @@ struct vx_plan {
 static void plan_free(vx_plan* p) {
     if (!p) return;
     void* ptrs[] = {p->twid, p->svec, p->sscal, p->sprob, p->sdat, p->q, p->qd, p->spec,
-                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft,
+                    p->sum, p->tgt, p->keys, p->real_out, p->rot_stage, p->dft, p->mr_w, p->mr_a,
                     p->q2, p->qd2, p->spec2, p->sum2, p->keys2, p->real_out2,
                     p->rs_maps, p->rs_best, p->rs_trans, p->rs_ncand, p->rs_cand, p->rs_partial};
     for (void* x : ptrs)
@@ -2118,8 +2109,33 @@ extern "C" int vx_plan_info(const vx_plan* p, int64_t* info) {
     return VX_OK;
 }
 
+// (Re)build the radix-split MFMA x pass's A operands for the plan's support box.
+static int ensure_mr(vx_plan* p) {
+    if (p->xvariant != 4 || (p->mr_a && p->mr_lo == p->lo && p->mr_bw == p->w)) return VX_OK;
+    std::vector<uint32_t> tab;
+    int rc = xcorr_mr_tables(p->n, p->lo, p->w, tab);
+    if (rc != VX_OK) return rc;
+    constexpr size_t kCap = (size_t)4 * 4 * 2 * 64 * 4;   // KSF + KSI <= 4
+    if (!p->mr_a) {
+        if (hipMalloc((void**)&p->mr_a, kCap * sizeof(uint32_t)) != hipSuccess) {
+            (void)hipGetLastError();
+            p->mr_a = nullptr;
+            set_error("device allocation of the MFMA x-pass operands failed");
+            return VX_ERR_NOMEM;
+        }
+        p->bytes += kCap * sizeof(uint32_t);
+    }
+    VX_HIP(hipDeviceSynchronize());   // no launch in flight may still read the old operands
+    VX_HIP(hipMemcpy(p->mr_a, tab.data(), tab.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    p->mr_lo = p->lo;
+    p->mr_bw = p->w;
+    return VX_OK;
+}
+
 // (Re)build the MFMA x pass's DFT matrices for the plan's support box.
 static int ensure_dft(vx_plan* p) {
+    const int rc = ensure_mr(p);
+    if (rc != VX_OK) return rc;
     if (p->xvariant != 2 || (p->dft && p->dft_lo == p->lo && p->dft_w == p->w)) return VX_OK;
     const int n = p->n, NP = 16 * ((n + 15) / 16);
     const int KP = (p->w + 15) / 16 * 16;
@@ -2299,7 +2315,28 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
     VX_REQUIRE(p, VX_ERR_ARG, "NULL plan");
     switch (key) {
         case VX_TUNE_XPASS_VARIANT:
-            VX_REQUIRE(value >= 0 && value <= 3, VX_ERR_ARG, "x-pass variant must be 0, 1, 2 or 3");
+            VX_REQUIRE(value >= 0 && value <= 4, VX_ERR_ARG, "x-pass variant must be 0, 1, 2, 3 or 4");
+            if (value == 4) {
+                int n1, n2;
+                VX_REQUIRE(p->G == 1, VX_ERR_UNSUPPORTED, "the MFMA radix-split x pass has one correlation group");
+                VX_REQUIRE(xcorr_mr_factor(p->n, &n1, &n2), VX_ERR_UNSUPPORTED,
+                           "grid edge %d has no factor N1 * N2 (N1 <= 32, N2 in 2, 3, 4, 5, 7, 8)", p->n);
+                if (!p->mr_w) {
+                    std::vector<float2> wt(p->n);
+                    for (int j = 0; j < p->n; ++j) {
+                        const double a = -2.0 * M_PI * (double)j / p->n;
+                        wt[j] = make_float2((float)cos(a), (float)sin(a));
+                    }
+                    if (hipMalloc((void**)&p->mr_w, p->n * sizeof(float2)) != hipSuccess) {
+                        (void)hipGetLastError();
+                        p->mr_w = nullptr;
+                        set_error("device allocation of the x-pass twiddles failed");
+                        return VX_ERR_NOMEM;
+                    }
+                    VX_HIP(hipMemcpy(p->mr_w, wt.data(), p->n * sizeof(float2), hipMemcpyHostToDevice));
+                    p->bytes += p->n * sizeof(float2);
+                }
+            }
             if (value == 2) {
                 VX_REQUIRE(p->G == 1, VX_ERR_UNSUPPORTED, "the MFMA x pass has one correlation group");
                 VX_REQUIRE((p->n + 15) / 16 <= kMfmaMaxNPT &&
@@ -2422,6 +2459,10 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
         const unsigned short* mats[4] = {p->dft, p->dft + p->dft_elems[0], p->dft + 2 * p->dft_elems[0],
                                          p->dft + 2 * p->dft_elems[0] + p->dft_elems[1]};
         rc = launch_xcorr_mfma(ws.spec, p->tgt, ws.sum, mats, n, nb, p->C, p->dft_kp, p->lo, p->w, p->xT, st);
+    } else if (p->xvariant == 4) {
+        VX_REQUIRE(p->mr_a && p->mr_lo == p->lo && p->mr_bw == p->w, VX_ERR_STATE,
+                   "MFMA x-pass operands do not match the support box");
+        rc = launch_xcorr_mr(ws.spec, p->tgt, ws.sum, p->mr_w, p->mr_a, n, nb, p->C, p->lo, p->w, st);
     } else {
         rc = launch_xcorr(ws.spec, p->tgt, ws.sum, p->fd, n, nb, p->C, p->G, grp_mask, p->lo, p->w, st,
                           p->xvariant, p->xT, p->static_plans != 0);
@@ -2543,8 +2584,15 @@ extern "C" int vx_rescore_f64(vx_plan* p, const double* tdata, const float* tvec
     if (p->mode == VX_MODE_VEC || prob) VX_REQUIRE(tvec && svec, VX_ERR_ARG, "tvec_dev and svec_dev required");
     if (prob) VX_REQUIRE(tprob && sprob, VX_ERR_ARG, "tprob_dev and sprob_dev required in prob mode");
     if (p->mode != VX_MODE_VEC && !prob) VX_REQUIRE(tdata, VX_ERR_ARG, "tdata_dev required in this mode");
-    VX_REQUIRE(p->xvariant != 2, VX_ERR_UNSUPPORTED, "FP64 re-scoring runs on the FFT x passes (variant 0, 1 or 3)");
     if (nrows == 0) return VX_OK;
+    // the candidates come from the FFT x pass's float32 maps whatever x pass
+    // the search runs (the MFMA forms round differently)
+    struct XpassGuard {
+        vx_plan* p;
+        int saved;
+        ~XpassGuard() { p->xvariant = saved; }
+    } guard{p, p->xvariant};
+    if (p->xvariant == 2 || p->xvariant == 4) p->xvariant = p->n * 4 <= 256 * kXcorrPF ? 3 : 1;
     hipStream_t st = (hipStream_t)stream;
     const int n = p->n;
     const size_t n3 = (size_t)n * n * n;
