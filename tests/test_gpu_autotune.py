@@ -1,8 +1,8 @@
 """The schedule autotuner on a slab grid and on a grid that only the pencil
 passes can hold: it returns a schedule the plan accepts, applies it, and the
 tuned plan scores the rows exactly as the default schedule does (every FFT
-schedule gives identical results; the MFMA x pass only enters within its
-1e-4 gate, checked against the FFT result)."""
+schedule gives identical results; the MFMA x passes only enter within
+their 1e-4 gate, checked against the FFT result)."""
 import numpy as np
 import pytest
 import torch
@@ -41,7 +41,7 @@ def test_autotune_applies_a_valid_schedule(gpu, n, batch):
     assert "xpass_variant" in sched and sched.get("streams") in (1, 2)
     b1, t1 = plan.score_rows(rot)
     b1, t1 = b1.cpu().numpy(), t1.cpu().numpy()
-    if sched["xpass_variant"] == 2 or sched.get("pencil") or sched.get("radix_order"):
+    if sched["xpass_variant"] in (2, 4) or sched.get("pencil") or sched.get("radix_order"):
         # held to the autotuner's 1e-4 gate
         assert np.max(np.abs(b1 - b0)) <= 1e-4 * np.max(np.abs(b0))
     else:
