@@ -108,16 +108,15 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     const int j0 = tile * kMrT;
     const bool colok = j0 + t < plane;
 
-    for (int j = tid; j < n; j += 256) sw[j] = wtab[j];
-    __syncthreads();
-
     // A operands (xcorr_mr_tables): fragment-ordered, read where used (they
     // stay in L1/L2; holding them would cost 8 (KSF + KSI) registers)
     const uint4* afw = atab + (size_t)wv * KSF * 128 + l;
     const uint4* aiw = atab + (size_t)4 * KSF * 128 + (size_t)wv * KSI * 128 + l;
 
-    // this thread's input elements: row r = g' + 16 i (x = N2 nlo + r), column t
-    const int rbase = tid >> 4;
+    // this thread's input elements: n1 = nlo + m, m = 4 wv + g + 16 kk (kk <
+    // KSF), every n2 (row x = N2 n1 + n2), column t.  A wave's 64 lanes then
+    // write one operand block, lane t + 16 wv, pair g: 64 distinct banks.
+    const int mq = 4 * wv + g;
     // Loads without branches, through buffer resources: a channel's input
     // resource spans rows [lo, lo + w) of the tile's columns, so rows outside
     // the support box read as zero (a row below lo gives a negative offset,
@@ -128,10 +127,11 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     auto load_channel = [&](int c, float2 (&xv)[kPer]) {
         const auto rs = mr_rsrc(spec + (((size_t)b * C + c) * n + lo) * (size_t)plane + j0,
                                 (uint32_t)w * plane * sizeof(float2));
-        const int r0 = N2 * nlo - lo + rbase;   // may be negative
+        const int r0 = N2 * (nlo + mq) - lo;   // may be negative
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const float2 v = mr_load(rs, (uint32_t)(((r0 + 16 * i) * plane + tcol) * (int)sizeof(float2)));
+            const int r = r0 + N2 * 16 * (i / N2) + i % N2;
+            const float2 v = mr_load(rs, (uint32_t)((r * plane + tcol) * (int)sizeof(float2)));
             xv[i] = make_float2(v.x * cm, v.y * cm);
         }
     };
@@ -155,6 +155,9 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     float2 xv[CH][kPer];
 #pragma unroll
     for (int c = 0; c < CH; ++c) load_channel(c, xv[c]);
+    // the twiddle table fills LDS while the inputs are in flight (the
+    // barrier below publishes both it and the maxima)
+    for (int j = tid; j < n; j += 256) sw[j] = wtab[j];
     float sc, isc;
     {
         float mx = 0.f;
@@ -182,10 +185,8 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
         uint32_t* bl = bh + kFwdWords;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
-            const int r = rbase + 16 * i;
-            const int m = r / N2, n2 = r - m * N2;
-            const int ks = m >> 4, gg = (m & 15) >> 2, jj = m & 3;
-            const int wi = (((ks * N2 + n2) * 64 + t + 16 * gg) << 2) + jj;
+            const int ks = i / N2, n2 = i % N2;
+            const int wi = (((ks * N2 + n2) * 64 + t + 16 * wv) << 2) + g;
             uint32_t h, lw;
             split_pair(xv[c][i].x * sc, xv[c][i].y * sc, h, lw);
             bh[wi] = h;
