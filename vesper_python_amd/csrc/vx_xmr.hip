@@ -192,9 +192,10 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
             bh[wi] = h;
             bl[wi] = lw;
         }
-        __syncthreads();
+        // the target's loads are in flight while the waves meet at the barrier
         float2 tv[2][N2];
         load_target(c, tv);
+        __syncthreads();
         f32x4_t acc[N2];
 #pragma unroll
         for (int k = 0; k < N2; ++k) acc[k] = f32x4_t{0.f, 0.f, 0.f, 0.f};
