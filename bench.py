@@ -57,7 +57,8 @@ def _stage_bytes(n: int, w: int, b: int) -> dict:
 
 PEAK_BF16_TFLOPS = 2500.0   # MI355X dense bf16 / fp16 MFMA
 
-PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "xcorr_pair200_pmc.md")
+PMC_SUMMARIES = [os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", f)
+                 for f in ("xcorr_pair200_pmc.md", "xcorr_mr_pair200.md")]
 
 
 def pmc_traffic(workload: str, batch: int, schedule, box_width: int):
@@ -69,17 +70,18 @@ def pmc_traffic(workload: str, batch: int, schedule, box_width: int):
         schedule = {"xpass_variant": 3, "xpass_tile": 8}
     want = {"workload": workload, "batch": str(batch), "variant": str(schedule.get("xpass_variant")),
             "tile": str(schedule.get("xpass_tile")), "box": str(box_width)}
-    try:
-        with open(PMC_SUMMARY) as f:
-            for line in f:
-                line = line.strip().strip("`")
-                if not line.startswith("traffic:"):
-                    continue
-                kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
-                if all(kv.get(k) == v for k, v in want.items()):
-                    return int(kv["bytes"])
-    except OSError:
-        pass
+    for path in PMC_SUMMARIES:
+        try:
+            with open(path) as f:
+                for line in f:
+                    line = line.strip().strip("`")
+                    if not line.startswith("traffic:"):
+                        continue
+                    kv = dict(t.split("=", 1) for t in line[len("traffic:"):].split() if "=" in t)
+                    if all(kv.get(k) == v for k, v in want.items()):
+                        return int(kv["bytes"])
+        except OSError:
+            pass
     return None
 
 
