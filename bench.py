@@ -901,14 +901,14 @@ def main():
         # 3 split products x N2 column blocks x (channels x forward K-steps +
         # inverse K-steps) v_mfma_f32_16x16x32_f16 of 16,384 flops each
         from vesper_python_amd.autotune import mr_factor
-        n1, n2 = mr_factor(n)
+        f1, f2 = mr_factor(n)
         lo_, w_ = info["box_lo"], info["box_width"]
-        nwin = (lo_ + w_ - 1) // n2 - lo_ // n2 + 1
-        ksf, ksi = (1 if nwin <= 16 else 2), (1 if n1 <= 16 else 2)
+        nwin = (lo_ + w_ - 1) // f2 - lo_ // f2 + 1
+        ksf, ksi = (1 if nwin <= 16 else 2), (1 if f1 <= 16 else 2)
         tiles = -(-(n * (n // 2 + 1)) // 16)
-        flops = args.batch * tiles * 4 * 3 * n2 * (3 * ksf + ksi) * 16384
+        flops = args.batch * tiles * 4 * 3 * f2 * (3 * ksf + ksi) * 16384
         tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
-        roofline["mfma"] = {"kernel": "xcorr (radix split, N = %d x %d)" % (n1, n2), "flops_per_launch": flops,
+        roofline["mfma"] = {"kernel": "xcorr (radix split, N = %d x %d)" % (f1, f2), "flops_per_launch": flops,
                             "achieved": round(tflops, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
                             "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
     if dom == "xcorr":
