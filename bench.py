@@ -266,6 +266,88 @@ def measure_prob(n1, n2, rot, batch, steps):
             "schedule": tuned["schedule"], "stage_ms_per_launch": {k: round(v / 8, 4) for k, v in stage.items()}}
 
 
+def _mfma_roofline(variant, n, info, batch, xcorr_ms):
+    """MFMA rate of an MFMA x pass from its per-launch time (ms), or None
+    for the FFT x passes."""
+    if variant == 2:
+        # split-bf16 whole-axis DFT: 3 products per real-form complex GEMM,
+        # forward [2NP x 2KP] x [2KP x plane] per channel, inverse [2NP x 2NP]
+        npad = 16 * ((n + 15) // 16)
+        kp = 16 * ((info["box_width"] + 15) // 16)
+        plane = n * (n // 2 + 1)
+        flops = 2 * 3 * batch * plane * (3 * (2 * npad) * (2 * kp) + (2 * npad) ** 2)
+        name = "xcorr"
+    elif variant == 4:
+        # radix split: per 16-column tile and rotation, 4 waves x 3 split
+        # products x N2 column blocks x (channels x forward K-steps + inverse
+        # K-steps) v_mfma_f32_16x16x32_f16 of 16,384 flops each
+        from vesper_python_amd.autotune import mr_factor
+        f1, f2 = mr_factor(n)
+        lo_, w_ = info["box_lo"], info["box_width"]
+        nwin = (lo_ + w_ - 1) // f2 - lo_ // f2 + 1
+        ksf, ksi = (1 if nwin <= 16 else 2), (1 if f1 <= 16 else 2)
+        tiles = -(-(n * (n // 2 + 1)) // 16)
+        flops = batch * tiles * 4 * 3 * f2 * (3 * ksf + ksi) * 16384
+        name = "xcorr (radix split, N = %d x %d)" % (f1, f2)
+    else:
+        return None
+    tflops = flops / (xcorr_ms / 1e3) / 1e12
+    return {"kernel": name, "flops_per_launch": flops, "achieved": round(tflops, 1), "peak": PEAK_BF16_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
+
+
+def measure_mixed(plan, rot, best, trans, n, batch, steps, plain_value, profile_rows, workload):
+    """The same exhaustive search with the radix-split MFMA x pass (the
+    length-N1 DFTs of N = N1 x N2 as split-f16 matrix products with fp32
+    accumulation, everything else fp32): rows/s over `steps` full passes
+    (HIP events), its scores against the fp32 run's (`best`, `trans`), and
+    the x pass's MFMA and HBM rates.  Leaves the plan on this schedule; None
+    where the grid edge has no radix split."""
+    import torch
+    from vesper_python_amd import engine
+    try:
+        plan.tune(xpass_variant=4, xpass_tile=0)
+    except engine.VesperError:
+        return None
+    R = int(rot.shape[0])
+    b2 = torch.empty_like(best)
+    t2 = torch.empty_like(trans)
+    plan.score_rows(rot, b2, t2)
+    engine.topn(b2, 10)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(st)
+    for _ in range(steps):
+        plan.score_rows(rot, b2, t2)
+        top = engine.topn(b2, 10)
+    e1.record(st)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    value = R / (ms / 1e3)
+    ref = best[:, 0].double()
+    rel = float(((b2[:, 0].double() - ref).abs().max() / ref.abs().max()).item())
+    teq = float((t2 == trans).all(dim=-1).double().mean().item())
+    top32 = engine.topn(best[:, 0].contiguous(), 10)
+    prows = min(R, max(batch, profile_rows // batch * batch))
+    stage_ms = plan.profile_rows(rot[:prows])
+    launches = (prows + batch - 1) // batch
+    info = plan.info()
+    xms = stage_ms["xcorr"] / launches
+    xbytes = _stage_bytes(n, info["box_width"], batch)["xcorr"]
+    return {"value": round(value, 2), "unit": "rotations/s", "ms_per_step": round(ms, 3),
+            "vs_plain_fp32": round(value / plain_value, 4),
+            "schedule": {"xpass_variant": 4},
+            "max_rel_score_diff_vs_fp32": rel, "translations_equal_vs_fp32": teq,
+            "top10_same_as_fp32": bool(np.array_equal(np.asarray(top), np.asarray(top32))),
+            "xcorr_ms_per_launch": round(xms, 4),
+            "xcorr_hbm": {"achieved": round(xbytes / (xms / 1e3) / 1e9, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                          "frac": round(xbytes / (xms / 1e3) / 1e9 / PEAK_HBM_GBS, 4)},
+            "mfma": _mfma_roofline(4, n, info, batch, xms),
+            "traffic": pmc_traffic(workload, batch, {"xpass_variant": 4, "xpass_tile": 0}, info["box_width"])}
+
+
 def measure_rescore(plan, n1, n2, angles, best):
     """The FP64 re-check on the benchmark pair: the rows near the top-10
     cutoff of the exhaustive float32 scores and the 3 x 3 x 3 refinement rows
@@ -834,8 +916,10 @@ def main():
     elif not args.no_tune:
         from vesper_python_amd import autotune
         # strided sample of the real rows; the schedule choice is cached per (device, N, mode, batch)
+        # the headline is the plain fp32 search: the MFMA x passes (split
+        # f16 / bf16 operands) are measured separately (measure_mixed)
         tuned = autotune.tune(plan, rot[::max(1, R // 256)].contiguous(), sample_rows=128, use_cache=False,
-                              verbose=True)
+                              verbose=True, mixed_precision=False)
     best = torch.empty((R, 1), dtype=torch.float32, device=dev)
     trans = torch.empty((R, 1, 3), dtype=torch.int32, device=dev)
     print(f"[bench] workload={args.workload} N={n} rows={R} batch={args.batch}", file=sys.stderr, flush=True)
@@ -886,34 +970,26 @@ def main():
                         "frac": round(value * row_bytes / 1e9 / PEAK_HBM_GBS, 4),
                         "model": "gather + z/y + x + y/z stage bytes of one launch / rows per launch"}
     sched = tuned["schedule"] if tuned else {}
-    if isinstance(sched, dict) and sched.get("xpass_variant") == 2:
-        # split-bf16 MFMA x pass: 3 products per real-form complex GEMM,
-        # forward [2NP x 2KP] x [2KP x plane] per channel, inverse [2NP x 2NP]
-        npad = 16 * ((n + 15) // 16)
-        kp = 16 * ((info["box_width"] + 15) // 16)
-        plane = n * (n // 2 + 1)
-        flops = 2 * 3 * args.batch * plane * (3 * (2 * npad) * (2 * kp) + (2 * npad) ** 2)
-        tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
-        roofline["mfma"] = {"kernel": "xcorr", "flops_per_launch": flops, "achieved": round(tflops, 1),
-                            "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s", "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
-    if isinstance(sched, dict) and sched.get("xpass_variant") == 4:
-        # radix-split MFMA x pass: per 16-column tile and rotation, 4 waves x
-        # 3 split products x N2 column blocks x (channels x forward K-steps +
-        # inverse K-steps) v_mfma_f32_16x16x32_f16 of 16,384 flops each
-        from vesper_python_amd.autotune import mr_factor
-        f1, f2 = mr_factor(n)
-        lo_, w_ = info["box_lo"], info["box_width"]
-        nwin = (lo_ + w_ - 1) // f2 - lo_ // f2 + 1
-        ksf, ksi = (1 if nwin <= 16 else 2), (1 if f1 <= 16 else 2)
-        tiles = -(-(n * (n // 2 + 1)) // 16)
-        flops = args.batch * tiles * 4 * 3 * f2 * (3 * ksf + ksi) * 16384
-        tflops = flops / (stage_ms["xcorr"] / launches / 1e3) / 1e12
-        roofline["mfma"] = {"kernel": "xcorr (radix split, N = %d x %d)" % (f1, f2), "flops_per_launch": flops,
-                            "achieved": round(tflops, 1), "peak": PEAK_BF16_TFLOPS, "unit": "TFLOP/s",
-                            "frac": round(tflops / PEAK_BF16_TFLOPS, 4)}
+    mf = _mfma_roofline(sched.get("xpass_variant") if isinstance(sched, dict) else None, n, info, args.batch,
+                        stage_ms["xcorr"] / launches)
+    if mf is not None:
+        roofline["mfma"] = mf
     if dom == "xcorr":
         roofline["traffic"] = pmc_traffic(args.workload, args.batch, tuned["schedule"] if tuned else None,
                                           info["box_width"])
+
+    # the batched mixed-precision search (radix-split MFMA x pass) and the
+    # learned proposal on it; then back to the all-fp32 schedule
+    mixed = None
+    if not args.rows and not args.schedule:
+        mixed = measure_mixed(plan, rot, best, trans, n, args.batch, args.steps, value, args.profile_rows,
+                              args.workload)
+    proposal = None
+    if not args.no_proposal and not args.rows:
+        proposal = measure_proposal(plan, n1, n2, w["A"], best, R, value)
+        proposal["schedule"] = "mixed precision (x-pass variant 4)" if mixed else "fp32"
+    if mixed is not None and tuned:
+        plan.tune(**tuned["schedule"])
 
     rescore = None
     aux = None
@@ -942,9 +1018,6 @@ def main():
     if not args.no_prob and not args.rows:
         prob = measure_prob(n1, n2, rot, max(8, args.batch // 2), 1)
 
-    proposal = None
-    if not args.no_proposal and not args.rows:
-        proposal = measure_proposal(plan, n1, n2, w["A"], best, R, value)
     cpu = cpu_all = None
     if not args.no_cpu:
         cpu = cpu_baseline(n1, n2, angles, args.cpu_seconds, 64)
@@ -965,7 +1038,7 @@ def main():
                       "schedule": tuned["schedule"] if tuned else "default"},
            "wall_s": round(wall, 3), "roofline": roofline, "cpu_baseline": cpu, "proposal": proposal,
            "prob_mode": prob, "interpolation": interp, "library_fft_pipeline": libfft, "uneven_pairs": pairs_m,
-           "fp64_recheck": rescore, "aux_rooflines": aux,
+           "fp64_recheck": rescore, "aux_rooflines": aux, "mixed_precision": mixed,
            "cli_end_to_end": cli_m}
     if cpu:
         cpu["parity_vs_gpu"] = baseline_parity(cpu, best, trans)

@@ -103,10 +103,13 @@ def _store(d: dict) -> None:
         pass
 
 
-def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True, verbose: bool = False) -> dict:
+def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True, verbose: bool = False,
+         mixed_precision: bool = True) -> dict:
     """Pick the fastest schedule for `plan` on rows `rot` (device [R, 9]);
-    applies it and returns {"schedule": ..., "ms": {...}}."""
-    key = _key(plan)
+    applies it and returns {"schedule": ..., "ms": {...}}.  With
+    mixed_precision=False the MFMA x passes (split bf16 / f16 operands) are
+    not candidates: every stage computes in fp32."""
+    key = _key(plan) + ("" if mixed_precision else "|fp32")
     with _lock:
         cache = _load() if use_cache else {}
     if use_cache and key in cache:
@@ -119,6 +122,8 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
     exact = None
     plan.tune(radix_order=0)
     for cand in candidates(plan.n, plan.groups):
+        if not mixed_precision and cand["xpass_variant"] in MFMA_VARIANTS:
+            continue
         try:
             plan.tune(**cand)
             got, _ = plan.score_rows(sample[:plan.batch])  # warm-up of this schedule
