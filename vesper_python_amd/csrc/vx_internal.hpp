@@ -15,6 +15,19 @@ constexpr size_t kMaxLds = 160 * 1024 - 512;
 
 struct RotF { float m[9]; };
 
+// Raw buffer resource over `bytes` bytes from `base` (uniform across the
+// workgroup); loads at offsets past the range return zero.  Third dword as
+// the gfx9 family (gfx950) takes it.  Offsets are 32-bit, in bytes.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float2 buf_load_f2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+    const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
+    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
+}
+
 // Workgroup -> (column tile, rotation) of the x pass.  Workgroups are handed
 // to the 8 XCDs round-robin by linear id, so XCD k = id % 8 is given the
 // contiguous tile range [k tpx, (k + 1) tpx) and, within it, every rotation

@@ -833,18 +833,25 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
     const int nt = SP::kStatic ? T : min(T, plane - j0);
     const int ne = n * nt;
     const float ntinv = 1.0f / (float)nt;
-    int off[PF];
+    // Byte offset of each slot's element (x, j0 + t) from row lo of its
+    // column tile: ((x - lo) plane + t) * 8, wrapping below row lo; slots
+    // without an element hold kNoSlot.  Loads go through buffer resources
+    // without branches: the query's resource spans the support box's rows,
+    // so rows outside it read as zero; the target's the whole channel (its
+    // offsets are these plus lo * plane * 8, which kNoSlot keeps out of range).
+    constexpr uint32_t kNoSlot = 0x80000000u;
+    uint32_t boff[PF];
 #pragma unroll
     for (int u = 0; u < PF; ++u) {
         const int e = threadIdx.x + u * NT;
-        off[u] = -1;
+        boff[u] = kNoSlot;
         if ((u + 1) * NT <= kFull || e < ne) {
             const int xx = qdiv<SP::kStatic>(e, nt, ntinv), t = e - xx * nt;
-            off[u] = xx * plane + t;
-            if (xx < lo || xx >= lo + w) off[u] |= 1 << 30;
+            boff[u] = (uint32_t)(((xx - lo) * plane + t) * (int)sizeof(float2));
         }
     }
-#define VX_SLOT(u) ((u + 1) * NT <= kFull || off[u] >= 0)
+    const uint32_t tshift = (uint32_t)(lo * plane * (int)sizeof(float2));
+#define VX_SLOT(u) ((u + 1) * NT <= kFull || boff[u] != kNoSlot)
     // channel order: group 0's channels, then group 1's (G == 2)
     auto in_group = [&](int c, int g) { return G == 1 || (int)((grp_mask >> c) & 1) == g; };
     auto next_channel = [&](int c, int g) {   // next channel of group g after c, else -1
@@ -853,18 +860,18 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
         return -1;
     };
     const float2* sb = spec + (size_t)b * C * n * (size_t)plane + j0;
+    const uint32_t box_bytes = (uint32_t)(w * plane * (int)sizeof(float2));
+    const uint32_t chan_bytes = (uint32_t)((n * plane - j0) * (int)sizeof(float2));
+    auto load_query = [&](int ch, float2 (&v)[PF]) {
+        const auto rs = buf_rsrc(sb + ((size_t)ch * n + lo) * plane, box_bytes);
+#pragma unroll
+        for (int u = 0; u < PF; ++u) v[u] = buf_load_f2(rs, boff[u]);
+    };
     float2 sv[PF], tv[PF], acc[PF];
     int c = next_channel(-1, 0);
     int gc = 0;   // group of the prefetched channel
     if (c < 0 && G == 2) { c = next_channel(-1, 1); gc = 1; }
-    if (c >= 0) {
-        const float2* sn = sb + (size_t)c * n * plane;
-#pragma unroll
-        for (int u = 0; u < PF; ++u) {
-            sv[u] = make_float2(0.f, 0.f);
-            if (VX_SLOT(u) && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
-        }
-    }
+    if (c >= 0) load_query(c, sv);
 #pragma unroll 1
     for (int g = 0; g < G; ++g) {
 #pragma unroll
@@ -877,18 +884,12 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 #pragma unroll
             for (int u = 0; u < PF; ++u)
                 if (VX_SLOT(u)) A[threadIdx.x + u * NT] = sv[u];
-            const float2* tc = tgt + (size_t)c * n * plane + j0;
+            {
+                const auto rt = buf_rsrc(tgt + (size_t)c * n * plane + j0, chan_bytes);
 #pragma unroll
-            for (int u = 0; u < PF; ++u)
-                if (VX_SLOT(u)) tv[u] = tc[off[u] & ~(1 << 30)];
-            if (pc >= 0) {
-                const float2* sn = sb + (size_t)pc * n * plane;
-#pragma unroll
-                for (int u = 0; u < PF; ++u) {
-                    sv[u] = make_float2(0.f, 0.f);
-                    if (VX_SLOT(u) && !(off[u] & (1 << 30))) sv[u] = sn[off[u]];
-                }
+                for (int u = 0; u < PF; ++u) tv[u] = buf_load_f2(rt, boff[u] + tshift);
             }
+            if (pc >= 0) load_query(pc, sv);
             __syncthreads();
             const float2* r = tile_fft<SP, RM, NT>(A, Bf, fd, nt);
 #pragma unroll
@@ -911,7 +912,7 @@ k_xcorr_reg_pf(const float2* __restrict__ spec, const float2* __restrict__ tgt, 
 #pragma unroll
         for (int u = 0; u < PF; ++u) {
             const int e = threadIdx.x + u * NT;
-            if (VX_SLOT(u)) dst[off[u] & ~(1 << 30)] = cconj(r[e]);
+            if (VX_SLOT(u)) dst[(boff[u] + tshift) / sizeof(float2)] = cconj(r[e]);
         }
         __syncthreads();
     }

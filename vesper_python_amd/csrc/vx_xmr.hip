@@ -69,19 +69,6 @@ __device__ __forceinline__ void tile_scale(float mx, float& sc, float& isc) {
     isc = ldexpf(1.f, e - kMrHeadroom);
 }
 
-typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
-
-// Raw buffer resource over `bytes` bytes from `base` (uniform); loads past
-// the range return zero.  Third dword as the gfx9 family (gfx950) takes it.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t mr_rsrc(const void* base, uint32_t bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
-}
-
-__device__ __forceinline__ float2 mr_load(__amdgpu_buffer_rsrc_t r, uint32_t off) {
-    const u32x2_t v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0);
-    return make_float2(__uint_as_float(v.x), __uint_as_float(v.y));
-}
-
 __device__ __forceinline__ float wave_max(float v) {
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
     return v;
@@ -125,13 +112,13 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     const int tcol = colok ? t : 0;
     const float cm = colok ? 1.f : 0.f;
     auto load_channel = [&](int c, float2 (&xv)[kPer]) {
-        const auto rs = mr_rsrc(spec + (((size_t)b * C + c) * n + lo) * (size_t)plane + j0,
+        const auto rs = buf_rsrc(spec + (((size_t)b * C + c) * n + lo) * (size_t)plane + j0,
                                 (uint32_t)w * plane * sizeof(float2));
         const int r0 = N2 * (nlo + mq) - lo;   // may be negative
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const int r = r0 + N2 * 16 * (i / N2) + i % N2;
-            const float2 v = mr_load(rs, (uint32_t)((r * plane + tcol) * (int)sizeof(float2)));
+            const float2 v = buf_load_f2(rs, (uint32_t)((r * plane + tcol) * (int)sizeof(float2)));
             xv[i] = make_float2(v.x * cm, v.y * cm);
         }
     };
@@ -139,13 +126,13 @@ k_xcorr_mr(const float2* __restrict__ spec, const float2* __restrict__ tgt, floa
     // rows k1 + N1 k2 of a channel's conjugated target at this lane's column
     // (rows past the channel read as zero; rows of k1 >= N1 are never used)
     auto load_target = [&](int c, float2 (&tv)[2][N2]) {
-        const auto rt = mr_rsrc(tgt + (size_t)c * n * plane + j0, (uint32_t)(n * plane - j0) * sizeof(float2));
+        const auto rt = buf_rsrc(tgt + (size_t)c * n * plane + j0, (uint32_t)(n * plane - j0) * sizeof(float2));
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const int k1 = 8 * wv + 2 * g + s;
 #pragma unroll
             for (int k2 = 0; k2 < N2; ++k2)
-                tv[s][k2] = mr_load(rt, (uint32_t)(((k1 + N1 * k2) * plane + tcol) * (int)sizeof(float2)));
+                tv[s][k2] = buf_load_f2(rt, (uint32_t)(((k1 + N1 * k2) * plane + tcol) * (int)sizeof(float2)));
         }
     };
 
