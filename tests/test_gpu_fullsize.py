@@ -25,6 +25,7 @@ import torch
 
 from oracle import vesper_ref as ref
 from oracle.build_c import fastvec_c
+from oracle_pool import OraclePool
 from vesper_python_amd import _native as nat
 from vesper_python_amd import engine, synth
 from vesper_python_amd import VESPER_1_prob_new as V
@@ -64,17 +65,6 @@ def _gpu_search(n1, n2, mode, angles):
     return best[:, 0].cpu().numpy(), trans[:, 0].cpu().numpy(), rows
 
 
-def _oracle_row(tgrid, spec, n2, angle, mode):
-    """Unscaled float32 maximum, its translation, and the channel-summed map."""
-    nv, nd = ref.rot_mrc(np.asarray(n2.data, np.float64), n2.vec, angle, prec="ref32")
-    res = ref.fft_correlate(spec, ref.query_channels(nd, nv, mode, tgrid.ave), "ref32")
-    b, t = ref.find_best_trans_list(res)
-    s = res[0]
-    for r in res[1:]:
-        s = s + r
-    return float(b), tuple(t), s
-
-
 def _fp64_sum(tgrid, spec, n2, angle, mode):
     nv, nd = ref.rot_mrc(np.asarray(n2.data, np.float64), n2.vec, angle, prec="ref64")
     return sum(ref.fft_correlate(spec, ref.query_channels(nd, nv, mode, tgrid.ave), "ref64"))
@@ -86,14 +76,10 @@ def _check_search(n1, n2, mode):
     b, t, rows = _gpu_search(n1, n2, mode, angles)
     tgrid = _tgrid(n1)
     spec = ref.target_spectra(ref.target_channels(tgrid, mode))
-    oracle = {}
     sample = list(range(0, len(angles), SAMPLE_STRIDE))
-    for r in list(rows) + sample:
-        r = int(r)
-        if r in oracle:
-            continue
-        print("[case] oracle row", mode, r, flush=True)
-        oracle[r] = _oracle_row(tgrid, spec, n2, angles[r], mode)
+    want = list(dict.fromkeys([int(r) for r in rows] + sample))
+    with OraclePool(np.asarray(n2.data, np.float64), n2.vec, spec, tgrid.ave) as pool:
+        oracle = pool.v_rows(want, angles, mode, gts={r: t[r] for r in want}, label="pair200")
     ties = 0
     for r in rows[:10]:
         ob, ot, _ = oracle[int(r)]
@@ -110,12 +96,12 @@ def _check_search(n1, n2, mode):
         assert oracle[int(r)][0] <= s10 + RTOL * abs(s10), (mode, r, oracle[int(r)][0], s10)
     worst = 0.0
     for r in sample:
-        ob, ot, s = oracle[r]
+        ob, ot, at = oracle[r]
         rel = abs(b[r] - ob) / abs(ob)
         worst = max(worst, rel)
         assert rel <= RTOL, (mode, r, b[r], ob)
         gt = tuple(int(v) for v in t[r])
-        assert gt == ot or abs(float(s[gt]) - ob) <= RTOL * abs(ob), (mode, r, gt, ot)
+        assert gt == ot or abs(at - ob) <= RTOL * abs(ob), (mode, r, gt, ot)
         assert ob <= s10 + RTOL * abs(s10) or r in set(int(x) for x in rows[:10]), (mode, r, ob, s10)
     print("[sample]", mode, len(sample), "rows, worst rel", worst, flush=True)
     return b, t, rows, tgrid, spec, angles
@@ -190,22 +176,18 @@ def test_pair200_prob_mode_config3_generator(gpu, pair200, fastvec_ref):
     spec = ref.target_spectra([tv[..., 0], tv[..., 1], tv[..., 2]] + [tp[k] for k in range(4)])
     sdata = np.asarray(n2.data, np.float64)
     sample = list(range(0, len(angles), 4 * SAMPLE_STRIDE))
-    s10 = None
-    for r in [int(x) for x in rows] + sample:
-        print("[case] config3 oracle row", r, flush=True)
-        nv, nd, q1, q2, q3, q4 = ref.rot_mrc_prob(sdata, n2.vec, *[sp[k] for k in range(4)], angles[r],
-                                                  prec="ref32")
-        rv = ref.fft_correlate(spec[:3], [nv[..., 0], nv[..., 1], nv[..., 2]], "ref32")
-        rp = ref.fft_correlate(spec[3:], [q1, q2, q3, q4], "ref32")
-        ms, mt = ref.find_best_trans_mixed(rv, rp, alpha, vstd, vave, pstd, pave, prec="ref32")
-        vs, vt = ref.find_best_trans_list(rv)
-        ps, _ = ref.find_best_trans_list(rp)
+    top = [int(x) for x in rows]
+    want = list(dict.fromkeys(top + sample))
+    with OraclePool(sdata, n2.vec, spec, prob=[sp[k] for k in range(4)]) as pool:
+        oracle = pool.prob_rows(want, angles, alpha, vstd, vave, pstd, pave, label="config3")
+    s10 = min(oracle[r][0] for r in top)
+    for r in want:
+        ms, mt, vs, vt, ps = oracle[r]
         scale = max(abs(float(ms)), 1.0)
         assert abs(b[r, 2] - ms) <= RTOL * scale, (r, b[r, 2], ms)
         assert abs(b[r, 0] - vs) <= RTOL * abs(vs) and abs(b[r, 1] - ps) <= RTOL * abs(ps), (r, b[r], vs, ps)
-        if r in set(int(x) for x in rows):
+        if r in top:
             assert tuple(int(v) for v in t[r, 2]) == tuple(mt), (r, t[r, 2], mt)
             assert tuple(int(v) for v in t[r, 0]) == tuple(vt), (r, t[r, 0], vt)
-            s10 = float(ms) if s10 is None else min(s10, float(ms))
         else:
             assert float(ms) <= s10 + RTOL * max(abs(s10), 1.0), (r, ms, s10)

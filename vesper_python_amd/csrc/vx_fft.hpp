@@ -404,20 +404,34 @@ struct StaticTile {
 // ---------------------------------------------------------------------------
 // Compile-time slab FFTs: the in-place, register-resident stages of fft_lds
 // (up to ceil(MAXV / R) butterflies per thread, read all, synchronise,
-// write all) with R, P, the span and the twiddle offset as constants, in two
-// layouts:
-//   tile  (TILE): element stride NL, line stride 1, consecutive threads on
-//                 consecutive lines (b = line + i NL);
-//   lines:        element stride 1, line stride N, consecutive threads on
-//                 consecutive butterflies of a line (b = i + line T).
+// write all) with R, P, the span and the twiddle offset as constants, in
+// three layouts:
+//   kLayTile:    element stride NL, line stride 1, consecutive threads on
+//                consecutive lines (b = line + i NL);
+//   kLayLines:   element stride 1, line stride N, consecutive threads on
+//                consecutive butterflies of a line (b = i + line T);
+//   kLayLinesLF: element stride 1, line stride N, consecutive threads on
+//                consecutive lines (b = line + i nlines; nlines = NL when
+//                NL > 0, else the runtime count with its float reciprocal).
+// LDS holds 64 four-byte banks, a float2 two of them, so a half wave's 32
+// float2 accesses are conflict-free when they fall on 32 distinct bank
+// pairs.  In kLayLines a half wave that crosses from one line into the next
+// reads element i of line l + 1 at pair (N + i) mod 32, which collides with
+// the run of line l for every span T not congruent to N mod 32 (N = 175: T
+// = 35 and 25 both collide).  In kLayLinesLF the lanes of one butterfly
+// index sit N apart, and an odd N reaches 32 distinct pairs.
 // ---------------------------------------------------------------------------
 
-template <int N, int NT, int MAXV, bool TILE, int NL, int R, int P, int TW>
-__device__ __forceinline__ void slab_stage_static(float2* buf, const float2* __restrict__ tw, int nlines) {
+constexpr int kLayLines = 0, kLayTile = 1, kLayLinesLF = 2;
+
+template <int N, int NT, int MAXV, int LAY, int NL, int R, int P, int TW>
+__device__ __forceinline__ void slab_stage_static(float2* buf, const float2* __restrict__ tw, int nlines,
+                                                  float nlinv) {
+    constexpr bool TILE = LAY == kLayTile;
     constexpr int T = N / R;
     constexpr int MAXB = (MAXV + R - 1) / R;
     constexpr int ES = TILE ? NL : 1;
-    const int nb = TILE ? T * NL : T * nlines;
+    const int nb = TILE ? T * NL : T * (LAY == kLayLinesLF && NL > 0 ? NL : nlines);
     float2 v[MAXB][R];
     int obase[MAXB];
 #pragma unroll
@@ -429,6 +443,16 @@ __device__ __forceinline__ void slab_stage_static(float2* buf, const float2* __r
             if constexpr (TILE) {
                 i = b / NL;
                 ib = b;
+            } else if constexpr (LAY == kLayLinesLF) {
+                int line;
+                if constexpr (NL > 0) {
+                    i = b / NL;
+                    line = b - i * NL;
+                } else {
+                    i = fdiv(b, nlines, nlinv);
+                    line = b - imul(i, nlines);
+                }
+                ib = line * N + i;
             } else {
                 const int line = b / T;
                 i = b - line * T;
@@ -459,11 +483,11 @@ __device__ __forceinline__ void slab_stage_static(float2* buf, const float2* __r
     __syncthreads();
 }
 
-template <int N, int NT, int MAXV, bool TILE, int NL, int P, int TW, int R, int... Rest>
-__device__ __forceinline__ void slab_fft_static_rec(float2* buf, const float2* tw, int nlines) {
-    slab_stage_static<N, NT, MAXV, TILE, NL, R, P, TW>(buf, tw, nlines);
+template <int N, int NT, int MAXV, int LAY, int NL, int P, int TW, int R, int... Rest>
+__device__ __forceinline__ void slab_fft_static_rec(float2* buf, const float2* tw, int nlines, float nlinv) {
+    slab_stage_static<N, NT, MAXV, LAY, NL, R, P, TW>(buf, tw, nlines, nlinv);
     if constexpr (sizeof...(Rest) > 0)
-        slab_fft_static_rec<N, NT, MAXV, TILE, NL, P * R, TW + (R - 1) * P, Rest...>(buf, tw, nlines);
+        slab_fft_static_rec<N, NT, MAXV, LAY, NL, P * R, TW + (R - 1) * P, Rest...>(buf, tw, nlines, nlinv);
 }
 
 template <int NL, int... Rs>
@@ -498,12 +522,20 @@ struct StaticPlan {
     // the N/2 + 1 half-spectrum columns of a slab, tile layout
     template <int NT, int MAXV>
     __device__ __forceinline__ static void tile_inplace(float2* buf, const float2* tw) {
-        slab_fft_static_rec<kN, NT, MAXV, true, kN / 2 + 1, 1, 0, Rs...>(buf, tw, 0);
+        slab_fft_static_rec<kN, NT, MAXV, kLayTile, kN / 2 + 1, 1, 0, Rs...>(buf, tw, 0, 0.f);
     }
-    // nlines contiguous lines of length N
+    // nlines contiguous lines of length N, consecutive threads on a line's
+    // consecutive butterflies
     template <int NT, int MAXV>
     __device__ __forceinline__ static void lines_inplace(float2* buf, const float2* tw, int nlines) {
-        slab_fft_static_rec<kN, NT, MAXV, false, 1, 1, 0, Rs...>(buf, tw, nlines);
+        slab_fft_static_rec<kN, NT, MAXV, kLayLines, 1, 1, 0, Rs...>(buf, tw, nlines, 0.f);
+    }
+    // the same lines with consecutive threads on consecutive lines; NLC > 0
+    // fixes the line count at compile time, else nlines and its reciprocal
+    template <int NT, int MAXV, int NLC = 0>
+    __device__ __forceinline__ static void lines_inplace_lf(float2* buf, const float2* tw, int nlines) {
+        slab_fft_static_rec<kN, NT, MAXV, kLayLinesLF, NLC, 1, 0, Rs...>(buf, tw, nlines,
+                                                                          1.0f / (float)nlines);
     }
 };
 
@@ -519,10 +551,11 @@ __device__ __forceinline__ void slab_fft_tile(float2* buf, const FftDesc& fd, in
     else fft_lds<MAXV, RM>(buf, fd, nl, nl, 1, true);
 }
 
-// fft_lds(buf, fd, nlines, 1, n, false)
-template <class SP, int MAXV, unsigned RM, int NT>
+// fft_lds(buf, fd, nlines, 1, n, false); a compile-time plan runs the lines
+// with consecutive threads on consecutive lines (NLC > 0: nlines == NLC)
+template <class SP, int MAXV, unsigned RM, int NT, int NLC = 0>
 __device__ __forceinline__ void slab_fft_lines(float2* buf, const FftDesc& fd, int nlines) {
-    if constexpr (SP::kStatic) SP::template lines_inplace<NT, MAXV>(buf, fd.twid, nlines);
+    if constexpr (SP::kStatic) SP::template lines_inplace_lf<NT, MAXV, NLC>(buf, fd.twid, nlines);
     else fft_lds<MAXV, RM>(buf, fd, nlines, 1, fd.n, false);
 }
 

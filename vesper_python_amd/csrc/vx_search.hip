@@ -1287,7 +1287,7 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
             if (dst[m] >= 0) lds[dst[m]] = Zc[m];
         __syncthreads();
     }
-    slab_fft_lines<SP, MAXV, RM, NT>(lds, fd, npair);
+    slab_fft_lines<SP, MAXV, RM, NT, SP::kStatic ? (SP::kN + 1) / 2 : 0>(lds, fd, npair);
     // z[n] = conj(F): row 2l = F.x, row 2l+1 = -F.y
     float bv = -INFINITY;
     uint32_t bi = 0xffffffffu;
