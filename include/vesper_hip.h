@@ -127,6 +127,12 @@ int vx_plan_destroy(vx_plan* plan);
 #define VX_TUNE_RADIX_ORDER 9 /* order of the FFT stages: 0 (default) = radix 8, 4, 2 then the odd
                                  primes ascending; 1 = reversed; 2 = odd primes ascending first;
                                  3 = odd primes descending first.  Results equal to fp32 rounding. */
+#define VX_TUNE_SLAB_FUSED 10 /* 1 (default): on a compile-time plan the (y, z) slab kernels read
+                                 their first stage from HBM, fold the half-spectrum unpack / pack
+                                 into the stage after it and write (or reduce) their last stage
+                                 from registers; 0 = separate load, unpack/pack and store passes
+                                 through LDS.  Results equal to fp32 rounding (bit-identical
+                                 at 175). */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

@@ -28,18 +28,23 @@ def load():
     P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
     lib.fastvec_ref.argtypes = [P, I, I, I, D, P, I, D, P, D, P, P]
     lib.fastvec_ref.restype = I
+    lib.fastvec_ref_planes.argtypes = [P, I, I, I, D, P, I, D, P, D, P, P, I, I]
+    lib.fastvec_ref_planes.restype = I
     return lib
 
 
-def fastvec_c(lib, src, src_width, src_orig, n, dest_width, dest_orig, dreso):
-    """(data float64 [n]^3, vec float32 [n]^3 x 3) from the C restatement."""
+def fastvec_c(lib, src, src_width, src_orig, n, dest_width, dest_orig, dreso, planes=None):
+    """(data float64 [n]^3, vec float32 [n]^3 x 3) from the C restatement;
+    planes = (x0, x1) computes output planes x0 <= x < x1 only (the others
+    stay zero)."""
     s = np.ascontiguousarray(src, dtype=np.float32)
     so = np.ascontiguousarray(np.asarray(src_orig, dtype=np.float32).astype(np.float64))
     do = np.ascontiguousarray(np.asarray(dest_orig, dtype=np.float64))
     data = np.zeros((n, n, n), np.float64)
     vec = np.zeros((n, n, n, 3), np.float32)
-    rc = lib.fastvec_ref(s.ctypes.data, s.shape[0], s.shape[1], s.shape[2], float(np.float32(src_width)),
-                         so.ctypes.data, int(n), float(dest_width), do.ctypes.data, float(dreso),
-                         data.ctypes.data, vec.ctypes.data)
+    x0, x1 = (0, int(n)) if planes is None else (int(planes[0]), int(planes[1]))
+    rc = lib.fastvec_ref_planes(s.ctypes.data, s.shape[0], s.shape[1], s.shape[2], float(np.float32(src_width)),
+                                so.ctypes.data, int(n), float(dest_width), do.ctypes.data, float(dreso),
+                                data.ctypes.data, vec.ctypes.data, x0, x1)
     assert rc == 0
     return data, vec

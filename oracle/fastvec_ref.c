@@ -14,10 +14,12 @@
 #include <stdint.h>
 #include <string.h>
 
-int fastvec_ref(const float *src, int sx, int sy, int sz,
-                double src_width, const double *src_orig,
-                int n, double dest_width, const double *dest_orig,
-                double dreso, double *out_data, float *out_vec)
+/* Output planes x in [x0, x1) only (each voxel is independent of the
+ * others); the rest of out_data / out_vec is left zero. */
+int fastvec_ref_planes(const float *src, int sx, int sy, int sz,
+                       double src_width, const double *src_orig,
+                       int n, double dest_width, const double *dest_orig,
+                       double dreso, double *out_data, float *out_vec, int x0, int x1)
 {
     const double gstep = src_width;                 /* :183 */
     double fs = (dreso / gstep) * 0.5;              /* :184 */
@@ -29,7 +31,9 @@ int fastvec_ref(const float *src, int sx, int sy, int sz,
     memset(out_data, 0, n3 * sizeof(double));
     memset(out_vec, 0, n3 * 3 * sizeof(float));
 
-    for (int x = 0; x < n; ++x)
+    if (x0 < 0) x0 = 0;
+    if (x1 > n) x1 = n;
+    for (int x = x0; x < x1; ++x)
     for (int y = 0; y < n; ++y)
     for (int z = 0; z < n; ++z) {
         const int xyz[3] = {x, y, z};
@@ -81,4 +85,13 @@ int fastvec_ref(const float *src, int sx, int sy, int sz,
         for (int a = 0; a < 3; ++a) out_vec[o * 3 + a] = (float)(t[a] * rdvec);   /* :255 */
     }
     return 0;
+}
+
+int fastvec_ref(const float *src, int sx, int sy, int sz,
+                double src_width, const double *src_orig,
+                int n, double dest_width, const double *dest_orig,
+                double dreso, double *out_data, float *out_vec)
+{
+    return fastvec_ref_planes(src, sx, sy, sz, src_width, src_orig, n, dest_width, dest_orig, dreso,
+                              out_data, out_vec, 0, n);
 }

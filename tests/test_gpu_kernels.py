@@ -565,6 +565,54 @@ def test_static_plans_identical(gpu, n, mode):
         _check_rows(res[0][0][:2, 0], res[0][1][:2, 0], o32, o64, "n=128")
 
 
+@pytest.mark.parametrize("n,mode", [(175, nat.VX_MODE_VEC), (175, nat.VX_MODE_PROB), (175, nat.VX_MODE_CC),
+                                    (48, nat.VX_MODE_VEC), (64, nat.VX_MODE_PROB), (77, nat.VX_MODE_CC),
+                                    (96, nat.VX_MODE_VEC), (128, nat.VX_MODE_PROB), (162, nat.VX_MODE_VEC),
+                                    (104, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (200, nat.VX_MODE_PROB)])
+def test_fused_slabs_identical(gpu, n, mode):
+    """The slab kernels that read their first stage from HBM, fold the
+    half-spectrum unpack / pack into the next stage and reduce (or store)
+    their last stage from registers give the same scores and
+    translations to the separate load / unpack / store passes (bit-identical
+    at 175; elsewhere the FFT code may contract a*b + c differently in the
+    two kernels, so scores agree to fp32 rounding), in every slab
+    shape (8 x 512: 48, 64, 77; 16 x 1024: 96..175; 24 x 1024: 182, 200), with
+    odd and even edges, and in prob mode, whose mixing pass has the inverse
+    slab write both real maps."""
+    rng = np.random.default_rng(9100 + n + mode)
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    inner = np.linalg.norm(g, axis=-1) <= 0.22 * n
+    td, tv = _maps(rng, n, 0.03)
+    sd, sv = _maps(rng, n, 0.3)
+    sd[~inner] = 0
+    sv[~inner] = 0
+    angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 170], [10, 350, 45], [95, 275, 5]], np.float64)
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    prob = None
+    if mode == nat.VX_MODE_PROB:
+        tp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (td != 0)
+        sp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (sd != 0)
+        prob = (_dev(tp, torch.float32), _dev(sp, torch.float32))
+    res = []
+    for fused in (True, False):
+        _say("fused slabs", n, mode, fused)
+        plan = engine.SearchPlan(n, mode, batch=2)
+        plan.tune(slab_fused=fused)
+        plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), prob[0] if prob else None, 0.3)
+        plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), prob[1] if prob else None, 0.3)
+        if prob:
+            plan.set_prob_stats(0.4, 2.0, 0.1, 3.0, 0.2, enable=True)
+        b, t = plan.score_rows(rot)
+        res.append((b.cpu().numpy(), t.cpu().numpy()))
+        plan.close()
+    if n == 175:
+        assert np.array_equal(res[0][0], res[1][0]), (n, mode, np.abs(res[0][0] - res[1][0]).max())
+    else:
+        assert np.allclose(res[0][0], res[1][0], rtol=1e-6, atol=0), (n, mode)
+    assert np.array_equal(res[0][1], res[1][1]), (n, mode)
+
+
 @pytest.mark.parametrize("n,mode", [(35, nat.VX_MODE_VEC), (24, nat.VX_MODE_CC)])
 def test_xpass_variants_identical(gpu, n, mode):
     """The FFT x-pass kernels (register, LDS, register-prefetch; several

@@ -509,6 +509,23 @@ struct StaticPlan {
     static constexpr int kN = (Rs * ...);
     static constexpr unsigned kMask = (radix_bit(Rs) | ...);
     static constexpr unsigned long long kCode = radix_code<Rs...>();
+    static constexpr int kNS = sizeof...(Rs);
+    static constexpr int kR[kNS] = {Rs...};
+    // product of the radices before stage s, and the float2 offset of stage
+    // s's twiddles in the descriptor's table ([j - 1][k] per stage)
+    static constexpr int P(int s) {
+        int p = 1;
+        for (int t = 0; t < s; ++t) p *= kR[t];
+        return p;
+    }
+    static constexpr int TW(int s) {
+        int o = 0, p = 1;
+        for (int t = 0; t < s; ++t) {
+            o += (kR[t] - 1) * p;
+            p *= kR[t];
+        }
+        return o;
+    }
     // the same radices as a column-tile plan of NL lines
     template <int NL>
     using Tile = StaticTile<NL, Rs...>;
@@ -543,6 +560,31 @@ struct DynamicPlan {
     static constexpr bool kStatic = false;
     static constexpr int kN = 0;
 };
+
+// Stages S .. SEND - 1 of a compile-time plan, in place over a slab in
+// layout LAY (the kernels that fuse a stage with their loads, stores or
+// layout changes run the other stages through this).
+template <class SP, int NT, int MAXV, int LAY, int NL, int S, int SEND>
+__device__ __forceinline__ void slab_stages(float2* buf, const float2* tw, int nlines, float nlinv) {
+    if constexpr (S < SEND) {
+        slab_stage_static<SP::kN, NT, MAXV, LAY, NL, SP::kR[S], SP::P(S), SP::TW(S)>(buf, tw, nlines, nlinv);
+        slab_stages<SP, NT, MAXV, LAY, NL, S + 1, SEND>(buf, tw, nlines, nlinv);
+    }
+}
+
+// The twiddles (k = i mod P) and radix-R DFT of one butterfly of a stage
+// with P earlier radices and twiddles at offset TW: the arithmetic of
+// slab_stage_static, for stages whose loads and stores are the caller's.
+template <int R, int P, int TW>
+__device__ __forceinline__ void twiddle_dft(float2 (&v)[R], int i, const float2* __restrict__ tw) {
+    if constexpr (P > 1) {
+        const int k = (int)((unsigned)i % (unsigned)P);
+        const float2* w = tw + TW + k;
+#pragma unroll
+        for (int j = 1; j < R; ++j) v[j] = cmul(v[j], w[(j - 1) * P]);
+    }
+    dft<R>(v);
+}
 
 // fft_lds(buf, fd, nl, nl, 1, true) with nl = n / 2 + 1 columns
 template <class SP, int MAXV, unsigned RM, int NT>
