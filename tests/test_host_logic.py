@@ -125,7 +125,7 @@ def test_static_lengths_follow_the_descriptor_order(name, lo, hi):
     24 x 1024 one every such length from 181 to 200, the x-pass-only ones
     every such length from 201 to 256 (8-column tiles) and from 257 to 400
     (4-column tiles)."""
-    src = open(os.path.join(ROOT, "vesper_python_amd", "csrc", "vx_search.hip")).read()
+    src = open(os.path.join(ROOT, "vesper_python_amd", "csrc", "vx_plans.hpp")).read()
     m = re.search(r"using %s = PlanList<(.*?)>;" % name, src, re.S)
     assert m
     plans = [[int(v) for v in g.split(",")] for g in re.findall(r"StaticPlan<([0-9, ]+)>", m.group(1))]

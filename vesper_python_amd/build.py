@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 OBJDIR = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libvesper_hip.so")
-SOURCES = ["vx_search.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip",
+SOURCES = ["vx_search.hip", "vx_slab.hip", "vx_fastvec.hip", "vx_score.hip", "vx_mlp.hip", "vx_interp.hip",
            "vx_noalloc.hip", "vx_rescore.hip", "vx_propose.hip", "vx_xmr.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

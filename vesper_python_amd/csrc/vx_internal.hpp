@@ -51,6 +51,14 @@ int launch_gather(int kind, const float4* svec, const float2* sscal, const float
                   double* qd, unsigned long long* keys, int nkeys, int lo, int w, int slo, int sw,
                   hipStream_t st);
 
+// Fused slab kernels on compile-time plans (vx_slab.hip): each returns false
+// (nothing launched) where the grid edge has no compile-time plan in its
+// slab shape, else true with *rc the launch status.
+bool launch_fwd_fused(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                      hipStream_t st, int nt_pref, int* rc);
+bool launch_inv_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
+                      int nkeys, float* real_out, hipStream_t st, int nt_pref, int* rc);
+
 // Top-N selection (vx_score.hip): keys per ping-pong buffer, argument check,
 // and the selection on caller-provided buffers of topn_cap keys each.
 size_t topn_cap(int nrows, int topn);

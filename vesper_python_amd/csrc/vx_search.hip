@@ -30,6 +30,7 @@
 #include "vx_fft.hpp"
 #include "vx_internal.hpp"
 #include "vx_gather.hpp"
+#include "vx_plans.hpp"
 
 namespace vx {
 
@@ -61,76 +62,6 @@ static bool factorize(int n, std::vector<int>& r, int order = 0) {
         if (order == 1) std::reverse(r.begin(), r.end());
     }
     return m == 1 && (int)r.size() <= kMaxStages;
-}
-
-// True when the descriptor's stages are exactly SP's radices in SP's order
-// (the compile-time plan then reads the descriptor's twiddle table unchanged).
-template <class SP>
-static bool plan_matches(const FftDesc& fd) {
-    return fd.n == SP::kN && fd.rcode == SP::kCode;
-}
-
-// Grid edges with compile-time stage plans (radices in fft_desc_create's
-// order): kernels instantiated for these lengths carry every stage's radix,
-// span and twiddle offset as constants.  This list is every FFTW-fast length
-// (2^a 3^b 5^c 7^d 11^e 13^f, e + f <= 1) from 90 to 180, the lengths whose
-// (y, z) slab takes the 16 x 1024 workgroup shape; 175 = 5 * 5 * 7 is the
-// grid edge of a 200^3 map at 2 A with the reference's padding.
-template <class... Ps>
-struct PlanList {};
-using StaticLengths = PlanList<StaticPlan<2, 3, 3, 5>, StaticPlan<7, 13>, StaticPlan<8, 4, 3>,
-    StaticPlan<2, 7, 7>, StaticPlan<3, 3, 11>, StaticPlan<4, 5, 5>, StaticPlan<8, 13>, StaticPlan<3, 5, 7>,
-    StaticPlan<4, 3, 3, 3>, StaticPlan<2, 5, 11>, StaticPlan<8, 2, 7>, StaticPlan<3, 3, 13>,
-    StaticPlan<8, 3, 5>, StaticPlan<5, 5, 5>, StaticPlan<2, 3, 3, 7>, StaticPlan<8, 8, 2>,
-    StaticPlan<2, 5, 13>, StaticPlan<4, 3, 11>, StaticPlan<3, 3, 3, 5>, StaticPlan<4, 5, 7>,
-    StaticPlan<8, 2, 3, 3>, StaticPlan<3, 7, 7>, StaticPlan<2, 3, 5, 5>, StaticPlan<2, 7, 11>,
-    StaticPlan<4, 3, 13>, StaticPlan<8, 4, 5>, StaticPlan<2, 3, 3, 3, 3>, StaticPlan<3, 5, 11>,
-    StaticPlan<8, 3, 7>, StaticPlan<5, 5, 7>, StaticPlan<8, 2, 11>, StaticPlan<4, 3, 3, 5>,
-    StaticPlan<7, 5, 5>>;   // 175 with its radix-7 stage first (radix orders 1 and 3)
-
-// The FFTW-fast lengths from 16 to 89 with two stages or more: their slab
-// takes the 8 x 512 workgroup shape.
-using SmallLengths = PlanList<StaticPlan<8, 2>, StaticPlan<2, 3, 3>, StaticPlan<4, 5>, StaticPlan<3, 7>,
-    StaticPlan<2, 11>, StaticPlan<8, 3>, StaticPlan<5, 5>, StaticPlan<2, 13>, StaticPlan<3, 3, 3>,
-    StaticPlan<4, 7>, StaticPlan<2, 3, 5>, StaticPlan<8, 4>, StaticPlan<3, 11>, StaticPlan<5, 7>,
-    StaticPlan<4, 3, 3>, StaticPlan<3, 13>, StaticPlan<8, 5>, StaticPlan<2, 3, 7>, StaticPlan<4, 11>,
-    StaticPlan<3, 3, 5>, StaticPlan<8, 2, 3>, StaticPlan<7, 7>, StaticPlan<2, 5, 5>, StaticPlan<4, 13>,
-    StaticPlan<2, 3, 3, 3>, StaticPlan<5, 11>, StaticPlan<8, 7>, StaticPlan<4, 3, 5>, StaticPlan<3, 3, 7>,
-    StaticPlan<8, 8>, StaticPlan<5, 13>, StaticPlan<2, 3, 11>, StaticPlan<2, 5, 7>, StaticPlan<8, 3, 3>,
-    StaticPlan<3, 5, 5>, StaticPlan<7, 11>, StaticPlan<2, 3, 13>, StaticPlan<8, 2, 5>, StaticPlan<3, 3, 3, 3>,
-    StaticPlan<4, 3, 7>, StaticPlan<8, 11>>;
-
-// The FFTW-fast lengths from 181 to 200: their slab takes the 24 x 1024
-// workgroup shape (above 200 the slab exceeds LDS and the pencils run).
-using MediumLengths = PlanList<StaticPlan<2, 7, 13>, StaticPlan<3, 3, 3, 7>, StaticPlan<8, 8, 3>,
-                               StaticPlan<3, 5, 13>, StaticPlan<4, 7, 7>, StaticPlan<2, 3, 3, 11>, StaticPlan<8, 5, 5>>;
-
-// The FFTW-fast lengths from 201 to 256: the pencil passes carry their
-// (y, z) transforms, and the register x pass (N x 8 <= 2048) its compile-time
-// tile plan.
-using TileLengths = PlanList<StaticPlan<8, 2, 13>, StaticPlan<2, 3, 5, 7>, StaticPlan<8, 3, 3, 3>,
-    StaticPlan<4, 5, 11>, StaticPlan<8, 4, 7>, StaticPlan<3, 3, 5, 5>, StaticPlan<3, 7, 11>,
-    StaticPlan<2, 3, 3, 13>, StaticPlan<8, 2, 3, 5>, StaticPlan<3, 3, 3, 3, 3>, StaticPlan<5, 7, 7>,
-    StaticPlan<2, 5, 5, 5>, StaticPlan<4, 3, 3, 7>, StaticPlan<8, 8, 4>>;
-
-// The FFTW-fast lengths from 257 to 400 (the larger grids of uneven map
-// pairs): the register x pass takes them in 4-column tiles (N x 4 <= 2048)
-// with a compile-time tile plan.
-using LargeLengths = PlanList<StaticPlan<4, 5, 13>, StaticPlan<8, 3, 11>, StaticPlan<2, 3, 3, 3, 5>,
-    StaticPlan<3, 7, 13>, StaticPlan<5, 5, 11>, StaticPlan<8, 5, 7>, StaticPlan<8, 4, 3, 3>, StaticPlan<2, 3, 7, 7>,
-    StaticPlan<3, 3, 3, 11>, StaticPlan<4, 3, 5, 5>, StaticPlan<4, 7, 11>, StaticPlan<8, 3, 13>,
-    StaticPlan<3, 3, 5, 7>, StaticPlan<8, 8, 5>, StaticPlan<4, 3, 3, 3, 3>, StaticPlan<5, 5, 13>,
-    StaticPlan<2, 3, 5, 11>, StaticPlan<8, 2, 3, 7>, StaticPlan<7, 7, 7>, StaticPlan<2, 5, 5, 7>,
-    StaticPlan<3, 3, 3, 13>, StaticPlan<8, 4, 11>, StaticPlan<8, 3, 3, 5>, StaticPlan<4, 7, 13>,
-    StaticPlan<3, 5, 5, 5>, StaticPlan<2, 3, 3, 3, 7>, StaticPlan<8, 8, 2, 3>, StaticPlan<5, 7, 11>,
-    StaticPlan<2, 3, 5, 13>, StaticPlan<8, 7, 7>, StaticPlan<4, 3, 3, 11>, StaticPlan<8, 2, 5, 5>>;
-
-// Calls f(SP{}) for the listed plan whose stages are the descriptor's; false if none is.
-template <class F, class... Ps>
-static bool with_static_plan(const FftDesc& fd, PlanList<Ps...>, F&& f) {
-    bool hit = false;
-    ((hit = hit || (plan_matches<Ps>(fd) && (f(Ps{}), true))), ...);
-    return hit;
 }
 
 int fft_desc_create(int n, FftDesc* d, float2** twid_out, int order) {
@@ -176,81 +107,6 @@ int fft_desc_create(int n, FftDesc* d, float2** twid_out, int order) {
 // ---------------------------------------------------------------------------
 // Rotate-and-gather
 // ---------------------------------------------------------------------------
-
-// Gather the value(s) landing on destination (x, y, z) for rotation F.
-//   KIND_VEC:     svec {vx, vy, vz, data != 0}        -> out[0..2]
-//   KIND_SCALAR:  sscal {channel value, data != 0}    -> out[0]
-//   KIND_PROB:    svec as above + sprob {p1..p4} each masked by its own != 0
-//                                                     -> out[0..6]
-//   KIND_DATA64:  sdat (float64, masked by != 0)      -> outd
-// svec / sscal / sprob hold only the support box [lo, lo + w)^3 of the source
-// (x-major, edge w): every voxel with non-zero density or probability lies
-// inside it, so a rotated position outside reads nothing, as a zero voxel
-// would.  sdat holds the whole grid.
-template <int KIND>
-__device__ __forceinline__ void gather_voxel(int x, int y, int z, int n, int lo, int w, const float* F, bool fma,
-                                             const float4* __restrict__ svec,
-                                             const float2* __restrict__ sscal,
-                                             const float4* __restrict__ sprob,
-                                             const double* __restrict__ sdat,
-                                             float* out, double* outd) {
-    constexpr int NOUT = (KIND == KIND_VEC) ? 3 : (KIND == KIND_PROB ? 7 : 1);
-#pragma unroll
-    for (int c = 0; c < NOUT; ++c) out[c] = 0.f;
-    *outd = 0.0;
-    int cx[2], cy[2], cz[2];
-    const int nx = dest_candidates(x, n, cx);
-    const int ny = dest_candidates(y, n, cy);
-    const int nz = dest_candidates(z, n, cz);
-    const float cent = 0.5f * (float)n;
-    const float fn = (float)n;
-    bool done_v = false;
-    bool done_p[4] = {false, false, false, false};
-    for (int a = 0; a < nz; ++a)
-    for (int b = 0; b < nx; ++b)
-    for (int c = 0; c < ny; ++c) {
-        const float n0 = (float)cx[b] - cent, n1 = (float)cy[c] - cent, n2 = (float)cz[a] - cent;
-        const float o0 = rot_coord(n0, n1, n2, F + 0, cent, fma);
-        const float o1 = rot_coord(n0, n1, n2, F + 3, cent, fma);
-        const float o2 = rot_coord(n0, n1, n2, F + 6, cent, fma);
-        if (!(o0 >= 0.f && o1 >= 0.f && o2 >= 0.f && o0 < fn && o1 < fn && o2 < fn)) continue;
-        size_t si;
-        if (KIND == KIND_DATA64) {
-            si = ((size_t)(int)o0 * n + (int)o1) * n + (int)o2;
-        } else {
-            const int b0 = (int)o0 - lo, b1 = (int)o1 - lo, b2 = (int)o2 - lo;
-            if ((unsigned)b0 >= (unsigned)w || (unsigned)b1 >= (unsigned)w || (unsigned)b2 >= (unsigned)w) continue;
-            si = ((size_t)b0 * w + b1) * w + b2;
-        }
-        if (KIND == KIND_VEC || KIND == KIND_PROB) {
-            if (!done_v) {
-                const float4 s = svec[si];
-                if (s.w != 0.f) {
-                    out[0] = rot_vec(s.x, s.y, s.z, F, 0, fma);
-                    out[1] = rot_vec(s.x, s.y, s.z, F, 1, fma);
-                    out[2] = rot_vec(s.x, s.y, s.z, F, 2, fma);
-                    done_v = true;
-                }
-            }
-            if (KIND == KIND_PROB) {
-                const float4 p = sprob[si];
-                const float pv[4] = {p.x, p.y, p.z, p.w};
-#pragma unroll
-                for (int k = 0; k < 4; ++k)
-                    if (!done_p[k] && pv[k] != 0.f) { out[3 + k] = pv[k]; done_p[k] = true; }
-                if (done_v && done_p[0] && done_p[1] && done_p[2] && done_p[3]) return;
-            } else if (done_v) {
-                return;
-            }
-        } else if (KIND == KIND_SCALAR) {
-            const float2 s = sscal[si];
-            if (s.y != 0.f) { out[0] = s.x; return; }
-        } else {
-            const double s = sdat[si];
-            if (s != 0.0) { *outd = s; return; }
-        }
-    }
-}
 
 template <int KIND>
 __global__ void __launch_bounds__(256)
@@ -452,132 +308,6 @@ k_fwd_zy(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, in
     for (int m = 0; m < MAXV; ++m) {
         const int e = threadIdx.x + m * nthr;
         if (e < n * nhz) out[e] = lds[e];
-    }
-}
-
-// k_fwd_zy on a compile-time plan with its boundary passes folded into the
-// stages next to them (same arithmetic, same results):
-//   * the first z stage reads its butterflies' inputs from the planes (two
-//     real rows as one complex line; consecutive threads on consecutive
-//     butterflies of a line, so the loads coalesce) instead of a load pass
-//     through LDS;
-//   * the half-spectrum unpack of each line pair happens inside the first y
-//     stage's loads: element (y, kz) is A or B of line y / 2, formed from Z[kz]
-//     and Z[n - kz], and rows outside the box's lines are zero without a read;
-//   * the last y stage, whose butterflies write the positions they read,
-//     stores to HBM instead of to LDS.
-// This takes three LDS passes and six barriers out of the slab.
-template <int MAXV, int NT, class SP>
-__global__ void __launch_bounds__(NT)
-k_fwd_zy_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int lo, int w) {
-    static_assert(SP::kStatic && SP::kNS >= 2, "the fused slab needs a compile-time plan of two stages or more");
-    constexpr int n = SP::kN;
-    constexpr int nhz = n / 2 + 1;
-    constexpr int NS = SP::kNS;
-    extern __shared__ float2 lds[];
-    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
-    const int x = lo + blockIdx.x;
-    const size_t plane = blockIdx.y;
-    const float* src = q + (plane * n + x) * (size_t)n * n;
-    const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
-    // z stage 0 (P = 1) from the planes; z outside [lo, lo + w) is zero
-    {
-        constexpr int R = SP::kR[0], T = n / R, MAXB = (MAXV + R - 1) / R;
-        const int nb = T * (l1 - l0);
-        float2 v[MAXB][R];
-        int ob[MAXB];
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            ob[m] = -1;
-            if (b < nb) {
-                const int lr = b / T, i = b - lr * T, l = l0 + lr;
-                const float* r0 = src + (size_t)(2 * l) * n;
-                const bool hb = 2 * l + 1 < n;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int z = i + j * T;
-                    float re = 0.f, im = 0.f;
-                    if (z >= lo && z < lo + w) {
-                        re = r0[z];
-                        if (hb) im = r0[n + z];
-                    }
-                    v[m][j] = make_float2(re, im);
-                }
-                dft<R>(v[m]);
-                ob[m] = l * n + i * R;
-            }
-        }
-        // nothing has read the slab's LDS yet: write without a barrier
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            if (ob[m] >= 0) {
-#pragma unroll
-                for (int j = 0; j < R; ++j) lds[ob[m] + j] = v[m][j];
-            }
-        }
-        __syncthreads();
-    }
-    slab_stages<SP, NT, MAXV, kLayLinesLF, 0, 1, NS>(lds + l0 * n, fd.twid, l1 - l0, 1.0f / (float)(l1 - l0));
-    // y stage 0 (P = 1) with the unpack Z = A + iB of line l = y / 2:
-    // A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
-    {
-        constexpr int R = SP::kR[0], T = n / R, NB = T * nhz, MAXB = (MAXV + R - 1) / R;
-        float2 v[MAXB][R];
-        int ob[MAXB];
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            ob[m] = -1;
-            if (b < NB) {
-                const int i = b / nhz, col = b - i * nhz;
-                const int cw = col == 0 ? 0 : n - col;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int y = i + j * T, l = y >> 1;
-                    float2 a = make_float2(0.f, 0.f);
-                    if (l >= l0 && l < l1) {
-                        const float2 zk = lds[l * n + col];
-                        const float2 wv = lds[l * n + cw];
-                        a = (y & 1) ? make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x))
-                                    : make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
-                    }
-                    v[m][j] = a;
-                }
-                dft<R>(v[m]);
-                ob[m] = b + (R - 1) * nhz * i;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            if (ob[m] >= 0) {
-#pragma unroll
-                for (int j = 0; j < R; ++j) lds[ob[m] + j * nhz] = v[m][j];
-            }
-        }
-        __syncthreads();
-    }
-    slab_stages<SP, NT, MAXV, kLayTile, nhz, 1, NS - 1>(lds, fd.twid, 0, 0.f);
-    // last y stage: butterfly (col, i < T = P) reads and writes rows i + j P
-    {
-        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R, NB = T * nhz;
-        constexpr int MAXB = (MAXV + R - 1) / R;
-        static_assert(T == P, "the last stage of a plan spans its earlier radices");
-        float2* out = spec + (plane * n + x) * (size_t)n * nhz;
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            if (b < NB) {
-                const int i = b / nhz;
-                float2 v[R];
-#pragma unroll
-                for (int j = 0; j < R; ++j) v[j] = lds[b + j * T * nhz];
-                twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
-#pragma unroll
-                for (int j = 0; j < R; ++j) out[b + j * P * nhz] = v[j];
-            }
-        }
     }
 }
 
@@ -1439,155 +1169,6 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
     }
 }
 
-// k_inv_yz on a compile-time plan with its boundary passes folded into the
-// stages next to them (same arithmetic, same results):
-//   * the first y stage reads its butterflies' inputs (conjugated) from HBM,
-//     coalesced along kz, instead of a load pass through LDS;
-//   * the Hermitian pack of row pairs (2l, 2l + 1) into one complex line
-//     happens inside the first z stage's loads (consecutive threads on
-//     consecutive butterflies of a line, so the row reads stay on distinct
-//     banks);
-//   * the last z stage, whose butterflies write the positions they read,
-//     reduces its outputs into the argmax (and writes the real maps when
-//     asked) from registers instead of through LDS.
-// Without real maps the last stage puts consecutive threads on consecutive
-// lines (conflict-free LDS reads at odd n); with them, on consecutive
-// butterflies of a line, so that the map stores coalesce along z.
-template <int MAXV, int NT, class SP>
-__global__ void __launch_bounds__(NT)
-k_inv_yz_fused(const float2* __restrict__ sum, FftDesc fdg, int G, unsigned long long* __restrict__ keys,
-               int nkeys, float* __restrict__ real_out) {
-    static_assert(SP::kStatic && SP::kNS >= 2, "the fused slab needs a compile-time plan of two stages or more");
-    constexpr int n = SP::kN;
-    constexpr int nhz = n / 2 + 1;
-    constexpr int npair = (n + 1) / 2;
-    constexpr int half = n / 2;
-    constexpr bool even = (n & 1) == 0;
-    constexpr int NS = SP::kNS;
-    extern __shared__ float2 lds[];
-    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
-    __shared__ float s_v[16];
-    __shared__ uint32_t s_i[16];
-    const int x = blockIdx.x;
-    const int bg = blockIdx.y;
-    const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
-    // y stage 0 (P = 1) from HBM: the conj-forward transform of the y-inverse
-    {
-        constexpr int R = SP::kR[0], T = n / R, NB = T * nhz, MAXB = (MAXV + R - 1) / R;
-        float2 v[MAXB][R];
-        int ob[MAXB];
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            ob[m] = -1;
-            if (b < NB) {
-                const int i = b / nhz;
-#pragma unroll
-                for (int j = 0; j < R; ++j) v[m][j] = cconj(src[b + j * T * nhz]);
-                dft<R>(v[m]);
-                ob[m] = b + (R - 1) * nhz * i;
-            }
-        }
-        // nothing has read the slab's LDS yet: write without a barrier
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            if (ob[m] >= 0) {
-#pragma unroll
-                for (int j = 0; j < R; ++j) lds[ob[m] + j * nhz] = v[m][j];
-            }
-        }
-        __syncthreads();
-    }
-    slab_stages<SP, NT, MAXV, kLayTile, nhz, 1, NS>(lds, fd.twid, 0, 0.f);
-    // z stage 0 (P = 1) with the pack: conj(Z), Z[k] = A[k] + i B[k] of rows
-    // 2l, 2l + 1 (lds holds their conjugates); Hermitian extension for
-    // k > n/2; imaginary parts of the DC (and Nyquist) bins dropped as c2r does
-    {
-        constexpr int R = SP::kR[0], T = n / R, NB = T * npair, MAXB = (MAXV + R - 1) / R;
-        float2 v[MAXB][R];
-        int ob[MAXB];
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            ob[m] = -1;
-            if (b < NB) {
-                const int l = b / T, i = b - l * T;
-                const bool hasb = (2 * l + 1) < n;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int k = i + j * T;
-                    if (k <= half) {
-                        float2 ra = lds[(2 * l) * nhz + k];
-                        float2 rb = hasb ? lds[(2 * l + 1) * nhz + k] : make_float2(0.f, 0.f);
-                        if (k == 0 || (even && k == half)) { ra.y = 0.f; rb.y = 0.f; }
-                        v[m][j] = make_float2(ra.x + rb.y, ra.y - rb.x);
-                    } else {
-                        const int kk = n - k;
-                        const float2 ra = lds[(2 * l) * nhz + kk];
-                        const float2 rb = hasb ? lds[(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
-                        v[m][j] = make_float2(ra.x - rb.y, -ra.y - rb.x);
-                    }
-                }
-                dft<R>(v[m]);
-                ob[m] = l * n + i * R;
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            if (ob[m] >= 0) {
-#pragma unroll
-                for (int j = 0; j < R; ++j) lds[ob[m] + j] = v[m][j];
-            }
-        }
-        __syncthreads();
-    }
-    slab_stages<SP, NT, MAXV, kLayLinesLF, npair, 1, NS - 1>(lds, fd.twid, npair, 0.f);
-    // last z stage + argmax: z[n] = conj(F), row 2l = F.x, row 2l+1 = -F.y
-    float bv = -INFINITY;
-    uint32_t bi = 0xffffffffu;
-    {
-        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R, NB = T * npair;
-        constexpr int MAXB = (MAXV + R - 1) / R;
-        static_assert(T == P, "the last stage of a plan spans its earlier radices");
-        const size_t n3 = (size_t)n * n * n;
-        const bool maps = real_out != nullptr;
-#pragma unroll
-        for (int m = 0; m < MAXB; ++m) {
-            const int b = (int)threadIdx.x + m * NT;
-            if (b < NB) {
-                int l, i;
-                if (maps) { l = b / T; i = b - l * T; }
-                else      { i = b / npair; l = b - i * npair; }
-                float2 v[R];
-#pragma unroll
-                for (int j = 0; j < R; ++j) v[j] = lds[l * n + i + j * T];
-                twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
-                const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
-                const bool hasb = 2 * l + 1 < n;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const uint32_t ia = row + i + j * P;
-                    const float va = v[j].x;
-                    if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
-                    if (maps) real_out[(size_t)bg * n3 + ia] = va;
-                    if (hasb) {
-                        const uint32_t ib = ia + n;
-                        const float vb = -v[j].y;
-                        if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
-                        if (maps) real_out[(size_t)bg * n3 + ib] = vb;
-                    }
-                }
-            }
-        }
-    }
-    block_argmax(bv, bi, s_v, s_i);
-    if (threadIdx.x == 0) {
-        const int b = bg / G, g = bg - b * G;
-        atomicMax(&keys[(size_t)b * nkeys + g], argmax_key(bv, bi));
-    }
-}
-
 // ---------------------------------------------------------------------------
 // Pencil passes for grids whose (y, z) slab does not fit in LDS
 // ---------------------------------------------------------------------------
@@ -1904,44 +1485,6 @@ __global__ void k_finalize(const unsigned long long* __restrict__ keys, int nb, 
 // Launch helpers
 // ---------------------------------------------------------------------------
 
-template <typename K>
-static int allow_lds(K kernel, size_t bytes) {
-    if (bytes > 65536) VX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
-    return VX_OK;
-}
-
-// Slab kernels: choose threads and per-thread value budget so that every
-// phase fits: z pass (npair x n), unpack (npair x nhz pairs, MAXV/2 each),
-// y pass (nhz x n), inverse pack (npair x n).  Variants (MAXV, threads):
-// (8, 512), (16, 512), (32, 512), (24, 1024).
-// Lengths whose stages are radix 5 and 7 only (35, 175, 245, ...) get
-// kernels compiled for just those radices: less code, fewer live scalars.
-constexpr unsigned kRadix57 = radix_bit(5) | radix_bit(7);
-
-static bool slab_fits(int n, int nhz, int mv, int t) {
-    const int npair = (n + 1) / 2;
-    return n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2);
-}
-
-// Slab workgroup shape: values per thread (MAXV) and threads.  nt_pref 0 or
-// 1024 takes the 16 x 1024 shape (code 17) where it fits and the slab needs
-// more than 8 x 512 (faster on MI355X: twice the waves per slab); 512 keeps
-// the 512-thread shapes.
-static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
-    if (nt_pref != 512 && slab_fits(n, nhz, 16, 1024) && !slab_fits(n, nhz, 8, 512)) {
-        *nthr = 1024; *maxv = 17; return;
-    }
-    const int npair = (n + 1) / 2;
-    const int opts[4][2] = {{8, 512}, {16, 512}, {32, 512}, {24, 1024}};
-    for (auto& o : opts) {
-        const int mv = o[0], t = o[1];
-        if (n * nhz <= t * mv && npair * n <= t * mv && npair * nhz <= t * ((mv + 1) / 2)) {
-            *nthr = t; *maxv = mv; return;
-        }
-    }
-    *nthr = 1024; *maxv = 0;
-}
-
 #define VX_SLAB_LAUNCH(KER, MV, NTH, ...)                                              \
     do {                                                                               \
         allow_lds(KER<MV, NTH>, lds);                                                  \
@@ -1951,11 +1494,6 @@ static void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
     do {                                                                               \
         allow_lds(KER<MV, NTH, RMV>, lds);                                             \
         hipLaunchKernelGGL((KER<MV, NTH, RMV>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
-    } while (0)
-#define VX_FUSED_LAUNCH(KER, MV, NTH, SPT, ...)                                        \
-    do {                                                                               \
-        allow_lds(KER<MV, NTH, SPT>, lds);                                             \
-        hipLaunchKernelGGL((KER<MV, NTH, SPT>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
     } while (0)
 #define VX_SLAB_LAUNCH_SP(KER, MV, NTH, RMV, SPT, ...)                                 \
     do {                                                                               \
@@ -1970,17 +1508,19 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     slab_shape(n, nhz, &nthr, &maxv, nt_pref);
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
+    if (use_static && fused) {
+        int rc;
+        if (launch_fwd_fused(q, spec, fd, n, nplanes, lo, w, st, nt_pref, &rc)) return rc;
+    }
     dim3 grid(w, nplanes);
     if (maxv == 8 && use_static && with_static_plan(fd, SmallLengths{}, [&](auto sp) {
             using SP = decltype(sp);
-            if (fused) VX_FUSED_LAUNCH(k_fwd_zy_fused, 8, 512, SP, q, spec, fd, lo, w);
-            else VX_SLAB_LAUNCH_SP(k_fwd_zy, 8, 512, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+            VX_SLAB_LAUNCH_SP(k_fwd_zy, 8, 512, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
         })) {
     } else if (maxv == 8) VX_SLAB_LAUNCH(k_fwd_zy, 8, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 17 && use_static && with_static_plan(fd, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
-                 if (fused) VX_FUSED_LAUNCH(k_fwd_zy_fused, 16, 1024, SP, q, spec, fd, lo, w);
-            else VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+                 VX_SLAB_LAUNCH_SP(k_fwd_zy, 16, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
              })) {
     } else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_fwd_zy, 16, 1024, kRadix57, q, spec, fd, n, nhz, lo, w);
@@ -1989,8 +1529,7 @@ int launch_fwd_zy(const float* q, float2* spec, const FftDesc& fd, int n, int np
     else if (maxv == 32) VX_SLAB_LAUNCH(k_fwd_zy, 32, 512, q, spec, fd, n, nhz, lo, w);
     else if (maxv == 24 && use_static && with_static_plan(fd, MediumLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
-                 if (fused) VX_FUSED_LAUNCH(k_fwd_zy_fused, 24, 1024, SP, q, spec, fd, lo, w);
-            else VX_SLAB_LAUNCH_SP(k_fwd_zy, 24, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
+                 VX_SLAB_LAUNCH_SP(k_fwd_zy, 24, 1024, SP::kMask, SP, q, spec, fd, n, nhz, lo, w);
              })) {
     } else               VX_SLAB_LAUNCH(k_fwd_zy, 24, 1024, q, spec, fd, n, nhz, lo, w);
     VX_LAUNCH_CHECK();
@@ -2005,17 +1544,19 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     slab_shape(n, nhz, &nthr, &maxv, nt_pref);
     const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
     VX_REQUIRE(maxv > 0 && lds <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d exceeds the LDS slab limit", n);
+    if (use_static && fused) {
+        int rc;
+        if (launch_inv_fused(sum, fd, n, nbg, G, keys, nkeys, real_out, st, nt_pref, &rc)) return rc;
+    }
     dim3 grid(n, nbg);
     if (maxv == 8 && use_static && with_static_plan(fd, SmallLengths{}, [&](auto sp) {
             using SP = decltype(sp);
-            if (fused) VX_FUSED_LAUNCH(k_inv_yz_fused, 8, 512, SP, sum, fd, G, keys, nkeys, real_out);
-            else VX_SLAB_LAUNCH_SP(k_inv_yz, 8, 512, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+            VX_SLAB_LAUNCH_SP(k_inv_yz, 8, 512, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
         })) {
     } else if (maxv == 8) VX_SLAB_LAUNCH(k_inv_yz, 8, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 17 && use_static && with_static_plan(fd, StaticLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
-                 if (fused) VX_FUSED_LAUNCH(k_inv_yz_fused, 16, 1024, SP, sum, fd, G, keys, nkeys, real_out);
-                 else VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+                 VX_SLAB_LAUNCH_SP(k_inv_yz, 16, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
              })) {
     } else if (maxv == 17 && fd.radix_mask == kRadix57)
         VX_SLAB_LAUNCH_RM(k_inv_yz, 16, 1024, kRadix57, sum, fd, n, nhz, G, keys, nkeys, real_out);
@@ -2024,8 +1565,7 @@ int launch_inv_yz(const float2* sum, const FftDesc& fd, int n, int nbg, int G,
     else if (maxv == 32) VX_SLAB_LAUNCH(k_inv_yz, 32, 512, sum, fd, n, nhz, G, keys, nkeys, real_out);
     else if (maxv == 24 && use_static && with_static_plan(fd, MediumLengths{}, [&](auto sp) {
                  using SP = decltype(sp);
-                 if (fused) VX_FUSED_LAUNCH(k_inv_yz_fused, 24, 1024, SP, sum, fd, G, keys, nkeys, real_out);
-                 else VX_SLAB_LAUNCH_SP(k_inv_yz, 24, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
+                 VX_SLAB_LAUNCH_SP(k_inv_yz, 24, 1024, SP::kMask, SP, sum, fd, n, nhz, G, keys, nkeys, real_out);
              })) {
     } else               VX_SLAB_LAUNCH(k_inv_yz, 24, 1024, sum, fd, n, nhz, G, keys, nkeys, real_out);
     VX_LAUNCH_CHECK();

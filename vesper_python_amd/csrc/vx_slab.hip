@@ -1,0 +1,386 @@
+// (y, z) slab kernels on compile-time FFT plans with their boundary passes
+// folded into the FFT stages next to them: the forward slab reading its first
+// stage from the query planes (k_fwd_zy_fused), and the inverse slab
+// reducing its last stage into the argmax (k_inv_yz_fused).  The
+// separate-pass forms and the runtime-plan kernels live in vx_search.hip;
+// these are the default on every grid edge with a compile-time plan
+// (VX_TUNE_SLAB_FUSED).
+// Their own translation unit, so that they compile beside the search
+// kernels.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <type_traits>
+
+#include "vx_common.hpp"
+#include "vx_fft.hpp"
+#include "vx_internal.hpp"
+#include "vx_plans.hpp"
+
+namespace vx {
+
+// The y half of a forward slab on a compile-time plan: lines [l0, l1) of
+// the slab hold the z spectra of row pairs (2l, 2l + 1); the first y stage
+// unpacks them (element (y, kz) is A or B of line y / 2, formed from Z[kz]
+// and Z[n - kz]; rows outside the lines are zero without a read), and the
+// last y stage, whose butterflies write the positions they read, stores the
+// slab's [ky][kz] spectrum to out.
+template <int MAXV, int NT, class SP>
+__device__ __forceinline__ void fwd_slab_y(float2* lds, const float2* __restrict__ tw, int l0, int l1,
+                                           float2* __restrict__ out) {
+    constexpr int n = SP::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int NS = SP::kNS;
+    // y stage 0 (P = 1) with the unpack Z = A + iB of line l = y / 2:
+    // A[k] = (Z[k] + conj Z[n-k]) / 2, B[k] = (Z[k] - conj Z[n-k]) / 2i
+    {
+        constexpr int R = SP::kR[0], T = n / R, NB = T * nhz, MAXB = (MAXV + R - 1) / R;
+        float2 v[MAXB][R];
+        int ob[MAXB];
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            ob[m] = -1;
+            if (b < NB) {
+                const int i = b / nhz, col = b - i * nhz;
+                const int cw = col == 0 ? 0 : n - col;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int y = i + j * T, l = y >> 1;
+                    float2 a = make_float2(0.f, 0.f);
+                    if (l >= l0 && l < l1) {
+                        const float2 zk = lds[l * n + col];
+                        const float2 wv = lds[l * n + cw];
+                        a = (y & 1) ? make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x))
+                                    : make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
+                    }
+                    v[m][j] = a;
+                }
+                dft<R>(v[m]);
+                ob[m] = b + (R - 1) * nhz * i;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            if (ob[m] >= 0) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) lds[ob[m] + j * nhz] = v[m][j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, MAXV, kLayTile, nhz, 1, NS - 1>(lds, tw, 0, 0.f);
+    // last y stage: butterfly (col, i < T = P) reads and writes rows i + j P
+    {
+        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R, NB = T * nhz;
+        constexpr int MAXB = (MAXV + R - 1) / R;
+        static_assert(T == P, "the last stage of a plan spans its earlier radices");
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            if (b < NB) {
+                const int i = b / nhz;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = lds[b + j * T * nhz];
+                twiddle_dft<R, P, SP::TW(S)>(v, i, tw);
+#pragma unroll
+                for (int j = 0; j < R; ++j) out[b + j * P * nhz] = v[j];
+            }
+        }
+    }
+}
+
+// k_fwd_zy on a compile-time plan with its boundary passes folded into the
+// stages next to them (same arithmetic, same results):
+//   * the first z stage reads its butterflies' inputs from the planes (two
+//     real rows as one complex line; consecutive threads on consecutive
+//     butterflies of a line, so the loads coalesce) instead of a load pass
+//     through LDS;
+//   * the half-spectrum unpack of each line pair happens inside the first y
+//     stage's loads: element (y, kz) is A or B of line y / 2, formed from Z[kz]
+//     and Z[n - kz], and rows outside the box's lines are zero without a read;
+//   * the last y stage, whose butterflies write the positions they read,
+//     stores to HBM instead of to LDS.
+// This takes three LDS passes and six barriers out of the slab.
+template <int MAXV, int NT, class SP>
+__global__ void __launch_bounds__(NT)
+k_fwd_zy_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int lo, int w) {
+    static_assert(SP::kStatic && SP::kNS >= 2, "the fused slab needs a compile-time plan of two stages or more");
+    constexpr int n = SP::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int NS = SP::kNS;
+    extern __shared__ float2 lds[];
+    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
+    const int x = lo + blockIdx.x;
+    const size_t plane = blockIdx.y;
+    const float* src = q + (plane * n + x) * (size_t)n * n;
+    const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
+    // z stage 0 (P = 1) from the planes; z outside [lo, lo + w) is zero
+    {
+        constexpr int R = SP::kR[0], T = n / R, MAXB = (MAXV + R - 1) / R;
+        const int nb = T * (l1 - l0);
+        float2 v[MAXB][R];
+        int ob[MAXB];
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            ob[m] = -1;
+            if (b < nb) {
+                const int lr = b / T, i = b - lr * T, l = l0 + lr;
+                const float* r0 = src + (size_t)(2 * l) * n;
+                const bool hb = 2 * l + 1 < n;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int z = i + j * T;
+                    float re = 0.f, im = 0.f;
+                    if (z >= lo && z < lo + w) {
+                        re = r0[z];
+                        if (hb) im = r0[n + z];
+                    }
+                    v[m][j] = make_float2(re, im);
+                }
+                dft<R>(v[m]);
+                ob[m] = l * n + i * R;
+            }
+        }
+        // nothing has read the slab's LDS yet: write without a barrier
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            if (ob[m] >= 0) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) lds[ob[m] + j] = v[m][j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, MAXV, kLayLinesLF, 0, 1, NS>(lds + l0 * n, fd.twid, l1 - l0, 1.0f / (float)(l1 - l0));
+    fwd_slab_y<MAXV, NT, SP>(lds, fd.twid, l0, l1, spec + (plane * n + x) * (size_t)n * nhz);
+}
+
+// k_inv_yz on a compile-time plan with its boundary passes folded into the
+// stages next to them (same arithmetic, same results):
+//   * the first y stage reads its butterflies' inputs (conjugated) from HBM,
+//     coalesced along kz, instead of a load pass through LDS;
+//   * the Hermitian pack of row pairs (2l, 2l + 1) into one complex line
+//     happens inside the first z stage's loads (consecutive threads on
+//     consecutive butterflies of a line, so the row reads stay on distinct
+//     banks);
+//   * the last z stage, whose butterflies write the positions they read,
+//     reduces its outputs into the argmax (and writes the real maps when
+//     asked) from registers instead of through LDS.
+// Without real maps the last stage puts consecutive threads on consecutive
+// lines (conflict-free LDS reads at odd n); with them, on consecutive
+// butterflies of a line, so that the map stores coalesce along z.
+template <int MAXV, int NT, class SP>
+__global__ void __launch_bounds__(NT)
+k_inv_yz_fused(const float2* __restrict__ sum, FftDesc fdg, int G, unsigned long long* __restrict__ keys,
+               int nkeys, float* __restrict__ real_out) {
+    static_assert(SP::kStatic && SP::kNS >= 2, "the fused slab needs a compile-time plan of two stages or more");
+    constexpr int n = SP::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int npair = (n + 1) / 2;
+    constexpr int half = n / 2;
+    constexpr bool even = (n & 1) == 0;
+    constexpr int NS = SP::kNS;
+    extern __shared__ float2 lds[];
+    const FftDesc fd = stage_twiddles(fdg, lds + n * nhz);
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    const int x = blockIdx.x;
+    const int bg = blockIdx.y;
+    const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
+    // y stage 0 (P = 1) from HBM: the conj-forward transform of the y-inverse
+    {
+        constexpr int R = SP::kR[0], T = n / R, NB = T * nhz, MAXB = (MAXV + R - 1) / R;
+        float2 v[MAXB][R];
+        int ob[MAXB];
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            ob[m] = -1;
+            if (b < NB) {
+                const int i = b / nhz;
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[m][j] = cconj(src[b + j * T * nhz]);
+                dft<R>(v[m]);
+                ob[m] = b + (R - 1) * nhz * i;
+            }
+        }
+        // nothing has read the slab's LDS yet: write without a barrier
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            if (ob[m] >= 0) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) lds[ob[m] + j * nhz] = v[m][j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, MAXV, kLayTile, nhz, 1, NS>(lds, fd.twid, 0, 0.f);
+    // z stage 0 (P = 1) with the pack: conj(Z), Z[k] = A[k] + i B[k] of rows
+    // 2l, 2l + 1 (lds holds their conjugates); Hermitian extension for
+    // k > n/2; imaginary parts of the DC (and Nyquist) bins dropped as c2r does
+    {
+        constexpr int R = SP::kR[0], T = n / R, NB = T * npair, MAXB = (MAXV + R - 1) / R;
+        float2 v[MAXB][R];
+        int ob[MAXB];
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            ob[m] = -1;
+            if (b < NB) {
+                const int l = b / T, i = b - l * T;
+                const bool hasb = (2 * l + 1) < n;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int k = i + j * T;
+                    if (k <= half) {
+                        float2 ra = lds[(2 * l) * nhz + k];
+                        float2 rb = hasb ? lds[(2 * l + 1) * nhz + k] : make_float2(0.f, 0.f);
+                        if (k == 0 || (even && k == half)) { ra.y = 0.f; rb.y = 0.f; }
+                        v[m][j] = make_float2(ra.x + rb.y, ra.y - rb.x);
+                    } else {
+                        const int kk = n - k;
+                        const float2 ra = lds[(2 * l) * nhz + kk];
+                        const float2 rb = hasb ? lds[(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+                        v[m][j] = make_float2(ra.x - rb.y, -ra.y - rb.x);
+                    }
+                }
+                dft<R>(v[m]);
+                ob[m] = l * n + i * R;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            if (ob[m] >= 0) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) lds[ob[m] + j] = v[m][j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, MAXV, kLayLinesLF, npair, 1, NS - 1>(lds, fd.twid, npair, 0.f);
+    // last z stage + argmax: z[n] = conj(F), row 2l = F.x, row 2l+1 = -F.y
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    {
+        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R, NB = T * npair;
+        constexpr int MAXB = (MAXV + R - 1) / R;
+        static_assert(T == P, "the last stage of a plan spans its earlier radices");
+        const size_t n3 = (size_t)n * n * n;
+        const bool maps = real_out != nullptr;
+#pragma unroll
+        for (int m = 0; m < MAXB; ++m) {
+            const int b = (int)threadIdx.x + m * NT;
+            if (b < NB) {
+                int l, i;
+                if (maps) { l = b / T; i = b - l * T; }
+                else      { i = b / npair; l = b - i * npair; }
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = lds[l * n + i + j * T];
+                twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
+                const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
+                const bool hasb = 2 * l + 1 < n;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const uint32_t ia = row + i + j * P;
+                    const float va = v[j].x;
+                    if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
+                    if (maps) real_out[(size_t)bg * n3 + ia] = va;
+                    if (hasb) {
+                        const uint32_t ib = ia + n;
+                        const float vb = -v[j].y;
+                        if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
+                        if (maps) real_out[(size_t)bg * n3 + ib] = vb;
+                    }
+                }
+            }
+        }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) {
+        const int b = bg / G, g = bg - b * G;
+        atomicMax(&keys[(size_t)b * nkeys + g], argmax_key(bv, bi));
+    }
+}
+
+#define VX_FUSED_LAUNCH(KER, MV, NTH, SPT, ...)                                        \
+    do {                                                                               \
+        allow_lds(KER<MV, NTH, SPT>, lds);                                             \
+        hipLaunchKernelGGL((KER<MV, NTH, SPT>), grid, dim3(NTH), lds, st, __VA_ARGS__); \
+    } while (0)
+
+// hipGetLastError after a launch, as VX_LAUNCH_CHECK reports it.
+static int launch_status() {
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) return VX_OK;
+    set_error("kernel launch failed: %s (%s)", hipGetErrorString(e), __FILE__);
+    return VX_ERR_HIP;
+}
+
+// k_fwd_zy_fused where the grid edge has a compile-time plan in its slab
+// shape; returns false (nothing launched) otherwise.
+bool launch_fwd_fused(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                      hipStream_t st, int nt_pref, int* rc) {
+    *rc = VX_OK;
+    const int nhz = n / 2 + 1;
+    int nthr, maxv;
+    slab_shape(n, nhz, &nthr, &maxv, nt_pref);
+    const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
+    if (maxv <= 0 || lds > kMaxLds) return false;
+    dim3 grid(w, nplanes);
+    bool hit = false;
+    if (maxv == 8)
+        hit = with_static_plan(fd, SmallLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_fwd_zy_fused, 8, 512, SP, q, spec, fd, lo, w);
+        });
+    else if (maxv == 17)
+        hit = with_static_plan(fd, StaticLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_fwd_zy_fused, 16, 1024, SP, q, spec, fd, lo, w);
+        });
+    else if (maxv == 24)
+        hit = with_static_plan(fd, MediumLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_fwd_zy_fused, 24, 1024, SP, q, spec, fd, lo, w);
+        });
+    if (hit) *rc = launch_status();
+    return hit;
+}
+
+// k_inv_yz_fused where the grid edge has a compile-time plan in its slab
+// shape; returns false (nothing launched) otherwise.
+bool launch_inv_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
+                      int nkeys, float* real_out, hipStream_t st, int nt_pref, int* rc) {
+    *rc = VX_OK;
+    const int nhz = n / 2 + 1;
+    int nthr, maxv;
+    slab_shape(n, nhz, &nthr, &maxv, nt_pref);
+    const size_t lds = ((size_t)n * nhz + fd.ntw) * sizeof(float2);
+    if (maxv <= 0 || lds > kMaxLds) return false;
+    dim3 grid(n, nbg);
+    bool hit = false;
+    if (maxv == 8)
+        hit = with_static_plan(fd, SmallLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_inv_yz_fused, 8, 512, SP, sum, fd, G, keys, nkeys, real_out);
+        });
+    else if (maxv == 17)
+        hit = with_static_plan(fd, StaticLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_inv_yz_fused, 16, 1024, SP, sum, fd, G, keys, nkeys, real_out);
+        });
+    else if (maxv == 24)
+        hit = with_static_plan(fd, MediumLengths{}, [&](auto sp) {
+            using SP = decltype(sp);
+            VX_FUSED_LAUNCH(k_inv_yz_fused, 24, 1024, SP, sum, fd, G, keys, nkeys, real_out);
+        });
+    if (hit) *rc = launch_status();
+    return hit;
+}
+
+}  // namespace vx
