@@ -258,6 +258,17 @@ int vx_rotate_map(int n, const float* rot_host9, const double* data_dev,
                   float* out_vec_dev, float* out_prob_dev, int flags, vx_stream stream);
 
 /*
+ * The float32 query planes that vx_score_rows correlates for the rotation
+ * rot_dev[9] (device): rot_mrc / rot_mrc_prob of the search map followed by
+ * the mode's query channels (VESPER_1_prob_new.py:279-354, :1211-1299,
+ * :546-558, :1434-1438, :1485-1497), formed by the search's own gather
+ * kernel.  out_dev holds C planes [C][n][n][n] (C as vx_mode_planes),
+ * zero outside the plan's support box.  Uses the plan's first workspace in
+ * Laplacian mode: do not overlap it with a search on the same plan.
+ */
+int vx_query_planes(vx_plan* plan, const float* rot_dev, float* out_dev, vx_stream stream);
+
+/*
  * Final FP64 re-scoring at a translation: get_score
  * (VESPER_1_prob_new.py:1302-1393), including its d2 = where(d2 <= 0, 0, d1)
  * line (:1357).  out_host[10] = {Nm, total, cc_raw, pcc_raw, dot_score,

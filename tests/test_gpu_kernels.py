@@ -71,6 +71,48 @@ def test_rotate_map_prob_bit_exact(gpu, n):
             assert np.array_equal(opn[k], e[2 + k]), k
 
 
+@pytest.mark.parametrize("n,mode,fma", [(24, "VecProduct", True), (25, "VecProduct", True),
+                                        (25, "VecProduct", False), (33, "CC", True), (30, "Overlap", True),
+                                        (27, "PCC", False), (39, "Laplacian", True), (21, "prob", True),
+                                        (22, "prob", False), (175, "VecProduct", True)])
+def test_query_planes_bit_exact(gpu, n, mode, fma):
+    """The planes the search itself correlates (its gather kernel, with the
+    support box and the compact source) equal the oracle's rot_mrc /
+    rot_mrc_prob followed by the mode's query channels, bit for bit: odd
+    edges take the centre planes' two candidates per axis, prob mode the
+    probability channels' own masks, and the density sits in a sphere so
+    that the box prunes."""
+    rng = np.random.default_rng(700 + n + 3 * int(fma))
+    c = n / 2
+    g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c
+    inner = np.linalg.norm(g, axis=-1) <= 0.3 * n
+    sd, sv = _maps(rng, n, 0.5)
+    sd[~inner] = 0
+    sv[~inner] = 0
+    code = nat.VX_MODE_PROB if mode == "prob" else nat.MODE_CODES[mode]
+    probs = None
+    if mode == "prob":
+        probs = [(rng.uniform(0, 1, (n, n, n)) * (rng.uniform(size=(n, n, n)) < 0.5) * inner).astype(np.float32)
+                 for _ in range(4)]
+    plan = engine.SearchPlan(n, code, batch=1, flags=0 if fma else nat.VX_FLAG_NO_FMA)
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32),
+                    _dev(np.stack(probs), torch.float32) if probs else None, 0.3)
+    angles = [[0, 0, 0], [90, 0, 0], [30, 60, 90], [200, 10, 170]] if n < 100 else [[30, 60, 90]]
+    for ang in angles + (rng.uniform(0, 360, (3, 3)).tolist() if n < 100 else []):
+        _say("query planes", n, mode, fma, ang)
+        got = plan.query_planes(torch.from_numpy(rotation_rows(ang)[0]).cuda()).cpu().numpy()
+        if mode == "prob":
+            e = ref.rot_mrc_prob(sd, sv, *probs, ang, fma=fma)
+            want = [e[0][..., 0], e[0][..., 1], e[0][..., 2]] + list(e[2:])
+        else:
+            nv, nd = ref.rot_mrc(sd, sv, ang, fma=fma)
+            want = ref.query_channels(nd, nv, mode, 0.3)
+        assert got.shape[0] == len(want)
+        for k, w in enumerate(want):
+            assert np.array_equal(got[k], np.asarray(w).astype(np.float32)), (n, mode, fma, ang, k)
+    plan.close()
+
+
 # ---------------------------------------------------------------- FFT search
 
 def _oracle_rows(tdata, tvec, sdata, svec, angles, mode, prec):

@@ -48,6 +48,7 @@ SIGNATURES = {
     "vx_set_search": (_I, [_P, _P, _P, _P, _D, _P]),
     "vx_set_prob_stats": (_I, [_P, _I, _D, _D, _D, _D, _D]),
     "vx_score_rows": (_I, [_P, _P, _I, _P, _P, _P]),
+    "vx_query_planes": (_I, [_P, _P, _P, _P]),
     "vx_score_rows_profile": (_I, [_P, _P, _I, _P, _P, _P, _P]),
     "vx_rescore_f64": (_I, [_P, _P, _P, _P, _D, _P, _P, _P, _P, _I, _D, _I, _P, _P, _P, _P]),
     "vx_topn": (_I, [_P, _I, _I, _I, _I, _P, _P]),

@@ -2588,3 +2588,27 @@ extern "C" int vx_rotate_map(int n, const float* rot9, const double* data, const
     VX_LAUNCH_CHECK();
     return VX_OK;
 }
+
+// The query planes vx_score_rows correlates for one rotation: the search's
+// own gather (k_gather, plus the Laplacian for that mode) into out_dev, with
+// zeros outside the plan's support box.
+extern "C" int vx_query_planes(vx_plan* p, const float* rot_dev, float* out_dev, vx_stream stream) {
+    VX_REQUIRE(p && rot_dev && out_dev, VX_ERR_ARG, "NULL argument");
+    VX_REQUIRE(p->have_search, VX_ERR_STATE, "search map not set");
+    hipStream_t st = (hipStream_t)stream;
+    const int n = p->n;
+    const size_t n3 = (size_t)n * n * n;
+    VX_HIP(hipMemsetAsync(out_dev, 0, (size_t)p->C * n3 * sizeof(float), st));
+    if (p->kind == KIND_DATA64) VX_HIP(hipMemsetAsync(p->qd, 0, n3 * sizeof(double), st));
+    int rc = launch_gather(p->kind, p->svec, p->sscal, p->sprob, p->sdat, rot_dev, n, 1,
+                           (p->flags & VX_FLAG_NO_FMA) ? 0 : 1, out_dev, p->qd, nullptr, 0, p->lo, p->w, p->src_lo,
+                           p->src_w, st);
+    if (rc != VX_OK) return rc;
+    if (p->kind == KIND_DATA64) {
+        hipLaunchKernelGGL(k_lap_to_q, dim3((unsigned)((n3 + 255) / 256), 1), dim3(256), 0, st, p->qd, out_dev, n);
+        VX_LAUNCH_CHECK();
+        // the search's own planes are cleared before its next batch
+        p->q_dirty = true;
+    }
+    return VX_OK;
+}

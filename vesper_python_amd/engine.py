@@ -285,6 +285,15 @@ class SearchPlan:
             nat.call("vx_score_rows", self._h, _p(rot), R, _p(best), _p(trans), _stream())
         return best, trans
 
+    def query_planes(self, rot: torch.Tensor) -> torch.Tensor:
+        """The float32 query planes [C, n, n, n] that score_rows correlates for
+        the rotation rot [9] (device float32), formed by the search's own
+        gather (vx_query_planes); zero outside the support box."""
+        C = int(self.info()["channels"])
+        out = torch.empty((C, self.n, self.n, self.n), dtype=torch.float32, device=rot.device)
+        nat.call("vx_query_planes", self._h, _p(rot.contiguous()), _p(out), _stream())
+        return out
+
     def rescore_f64(self, rot: torch.Tensor, target, search, ave: float = 0.0, window: float = 1e-5,
                     max_cand: int = 64):
         """FP64 scores and translations of the rows of rot [R, 9] (device
