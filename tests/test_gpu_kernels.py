@@ -610,7 +610,9 @@ def test_static_plans_identical(gpu, n, mode):
 @pytest.mark.parametrize("n,mode", [(175, nat.VX_MODE_VEC), (175, nat.VX_MODE_PROB), (175, nat.VX_MODE_CC),
                                     (48, nat.VX_MODE_VEC), (64, nat.VX_MODE_PROB), (77, nat.VX_MODE_CC),
                                     (96, nat.VX_MODE_VEC), (128, nat.VX_MODE_PROB), (162, nat.VX_MODE_VEC),
-                                    (104, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (200, nat.VX_MODE_PROB)])
+                                    (104, nat.VX_MODE_VEC), (182, nat.VX_MODE_CC), (200, nat.VX_MODE_PROB),
+                                    (208, nat.VX_MODE_VEC), (250, nat.VX_MODE_VEC), (270, nat.VX_MODE_CC),
+                                    (308, nat.VX_MODE_PROB)])
 def test_fused_slabs_identical(gpu, n, mode):
     """The slab kernels that read their first stage from HBM, fold the
     half-spectrum unpack / pack into the next stage and reduce (or store)
@@ -620,7 +622,8 @@ def test_fused_slabs_identical(gpu, n, mode):
     two kernels, so scores agree to fp32 rounding), in every slab
     shape (8 x 512: 48, 64, 77; 16 x 1024: 96..175; 24 x 1024: 182, 200), with
     odd and even edges, and in prob mode, whose mixing pass has the inverse
-    slab write both real maps."""
+    slab write both real maps.  Above 200 the same folds run in the pencil
+    passes (forward z, y both ways, inverse z with the argmax)."""
     rng = np.random.default_rng(9100 + n + mode)
     c = n / 2
     g = np.indices((n, n, n)).transpose(1, 2, 3, 0) - c

@@ -386,8 +386,12 @@ __device__ __forceinline__ float2* tile_fft_static_rec(float2* a, float2* b, con
     else return tile_fft_static_rec<N, NL, NT, P * R, TW + (R - 1) * P, Rest...>(b, a, tw);
 }
 
+template <int... Rs>
+struct StaticPlan;
+
 template <int NL, int... Rs>
 struct StaticTile {
+    using Plan = StaticPlan<Rs...>;
     static constexpr bool kStatic = true;
     static constexpr int kN = (Rs * ...);
     static constexpr int kNL = NL;
@@ -569,6 +573,19 @@ __device__ __forceinline__ void slab_stages(float2* buf, const float2* tw, int n
     if constexpr (S < SEND) {
         slab_stage_static<SP::kN, NT, MAXV, LAY, NL, SP::kR[S], SP::P(S), SP::TW(S)>(buf, tw, nlines, nlinv);
         slab_stages<SP, NT, MAXV, LAY, NL, S + 1, SEND>(buf, tw, nlines, nlinv);
+    }
+}
+
+// Stages S .. SEND - 1 of a compile-time plan over a column tile of NL
+// lines (element stride NL), ping-pong from a to b; returns the buffer that
+// holds the result (ends synchronised).
+template <class PL, int NL, int NT, int S, int SEND>
+__device__ __forceinline__ float2* tile_stages_pp(float2* a, float2* b, const float2* tw) {
+    if constexpr (S >= SEND) {
+        return a;
+    } else {
+        tile_stage_static<PL::kN, NL, NT, PL::kR[S], PL::P(S), PL::TW(S)>(a, b, tw);
+        return tile_stages_pp<PL, NL, NT, S + 1, SEND>(b, a, tw);
     }
 }
 

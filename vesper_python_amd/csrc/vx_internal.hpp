@@ -58,6 +58,14 @@ bool launch_fwd_fused(const float* q, float2* spec, const FftDesc& fd, int n, in
                       hipStream_t st, int nt_pref, int* rc);
 bool launch_inv_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
                       int nkeys, float* real_out, hipStream_t st, int nt_pref, int* rc);
+// The pencil passes' fused forms (grid edges 201..400); launch_axis_y_fused
+// covers the full column tiles (*full_tiles of them) and leaves the rest.
+bool launch_fwd_z_fused(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                        hipStream_t st, int* rc);
+bool launch_axis_y_fused(float2* buf, const FftDesc& fd, int n, int inverse, int ylo, int yw, int xlo, int nxs,
+                         int nplanes, hipStream_t st, int* full_tiles, int* rc);
+bool launch_inv_z_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
+                        int nkeys, float* real_out, hipStream_t st, int* rc);
 
 // Top-N selection (vx_score.hip): keys per ping-pong buffer, argument check,
 // and the selection on caller-provided buffers of topn_cap keys each.

@@ -118,4 +118,30 @@ inline void slab_shape(int n, int nhz, int* nthr, int* maxv, int nt_pref = 0) {
     *nthr = 1024; *maxv = 0;
 }
 
+// Values per thread of a pencil workgroup's in-place z pass on a
+// compile-time plan.
+constexpr int kPencilMaxV = 8;
+
+// Values per pencil workgroup: ~2k complex (32 KB of ping-pong LDS, four to
+// five 256-thread workgroups per CU).  At ~4k (two workgroups per CU) the
+// pencil searches ran 8-39 % slower (N = 208..343): too few waves to hide the
+// LDS latency of the stages; at ~1k the shorter lines per workgroup lost more
+// than the extra waves gained.
+constexpr int kPencilValues = 2048;
+
+inline int pencil_pairs(int n) {
+    int P = kPencilValues / n;
+    if (P < 1) P = 1;
+    if (P > 32) P = 32;
+    return P;
+}
+
+inline int pencil_T(int n) {
+    // columns per y-pass workgroup: a power of two (runs of 8 T bytes stay
+    // aligned to segments), at most 16
+    int T = 16;
+    while (T > 1 && T * n > kPencilValues) T >>= 1;
+    return T;
+}
+
 }  // namespace vx

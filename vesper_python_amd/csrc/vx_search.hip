@@ -1179,7 +1179,6 @@ k_inv_yz(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G,
 // z in [lo, lo + w).
 // With a compile-time plan (SP = StaticPlan<radices>) the lines transform in
 // place on registers (at most kPencilMaxV values per thread).
-constexpr int kPencilMaxV = 8;
 
 template <class SP = DynamicPlan>
 __global__ void __launch_bounds__(256)
@@ -1333,28 +1332,6 @@ k_inv_z(const float2* __restrict__ sum, FftDesc fdg, int n_, int nhz_, int G, in
     }
 }
 
-// Values per pencil workgroup: ~2k complex (32 KB of ping-pong LDS, four to
-// five 256-thread workgroups per CU).  At ~4k (two workgroups per CU) the
-// pencil searches ran 8-39 % slower (N = 208..343): too few waves to hide the
-// LDS latency of the stages; at ~1k the shorter lines per workgroup lost more
-// than the extra waves gained.
-constexpr int kPencilValues = 2048;
-
-static int pencil_pairs(int n) {
-    int P = kPencilValues / n;
-    if (P < 1) P = 1;
-    if (P > 32) P = 32;
-    return P;
-}
-
-static int pencil_T(int n) {
-    // columns per y-pass workgroup: a power of two (runs of 8 T bytes stay
-    // aligned to segments), at most 16
-    int T = 16;
-    while (T > 1 && T * n > kPencilValues) T >>= 1;
-    return T;
-}
-
 template <typename K>
 static int allow_lds_any(K kernel, size_t bytes) {
     if (bytes > 65536) VX_HIP(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
@@ -1366,7 +1343,7 @@ static int allow_lds_any(K kernel, size_t bytes) {
 // 201..256, 4-column above), a last partial tile and the other edges on the
 // runtime plan.
 static int launch_axis_y(float2* buf, const FftDesc& fd, int n, int inverse, int ylo, int yw, int xlo, int nxs,
-                         int nplanes, bool use_static, hipStream_t st) {
+                         int nplanes, bool use_static, hipStream_t st, bool fused = true) {
     const int nhz = n / 2 + 1;
     const int T = pencil_T(n);
     const size_t ly = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
@@ -1381,7 +1358,13 @@ static int launch_axis_y(float2* buf, const FftDesc& fd, int n, int inverse, int
                            full, nxs, total, 0);
     };
     bool hit = false;
-    if (use_static && full > 0 && T == 8)
+    if (use_static && fused && full > 0) {
+        int rc, nfull;
+        hit = launch_axis_y_fused(buf, fd, n, inverse, ylo, yw, xlo, nxs, nplanes, st, &nfull, &rc);
+        if (rc != VX_OK) return rc;
+    }
+    if (hit) {
+    } else if (use_static && full > 0 && T == 8)
         hit = with_static_plan(fd, TileLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 8>{}); });
     else if (use_static && full > 0 && T == 4)
         hit = with_static_plan(fd, LargeLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 4>{}); });
@@ -1398,12 +1381,19 @@ static int launch_axis_y(float2* buf, const FftDesc& fd, int n, int inverse, int
 }
 
 int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
-                      hipStream_t st, bool use_static = true) {
+                      hipStream_t st, bool use_static = true, bool fused = true) {
     const int nhz = n / 2 + 1;
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
     const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
+    if (use_static && fused) {
+        int rc;
+        if (launch_fwd_z_fused(q, spec, fd, n, nplanes, lo, w, st, &rc)) {
+            if (rc != VX_OK) return rc;
+            return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st, fused);
+        }
+    }
     const dim3 gz((l1 - l0 + P - 1) / P, w, nplanes);
     auto zgo = [&](auto sp) {
         using SP = decltype(sp);
@@ -1415,14 +1405,15 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
           (with_static_plan(fd, TileLengths{}, zgo) || with_static_plan(fd, LargeLengths{}, zgo))))
         zgo(DynamicPlan{});
     VX_LAUNCH_CHECK();
-    return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st);
+    return launch_axis_y(spec, fd, n, 0, 2 * l0, 2 * (l1 - l0), lo, w, nplanes, use_static, st, fused);
 }
 
 int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys, int nkeys,
-                      float* real_out, hipStream_t st, bool use_static = true) {
+                      float* real_out, hipStream_t st, bool use_static = true, bool fused = true) {
     const int nhz = n / 2 + 1;
-    int rc = launch_axis_y(sum, fd, n, 1, 0, n, 0, n, nbg, use_static, st);
+    int rc = launch_axis_y(sum, fd, n, 1, 0, n, 0, n, nbg, use_static, st, fused);
     if (rc != VX_OK) return rc;
+    if (use_static && fused && launch_inv_z_fused(sum, fd, n, nbg, G, keys, nkeys, real_out, st, &rc)) return rc;
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
@@ -2095,7 +2086,7 @@ extern "C" int vx_set_target(vx_plan* p, const double* tdata, const float* tvec,
     VX_LAUNCH_CHECK();
     // C planes of batch slot 0 -> spectra (x-slab pass), then x pass with conj.
     // spec holds B*C planes >= C planes.
-    int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st, p->static_plans != 0)
+    int rc = p->pencil ? launch_fwd_pencil(p->q, p->spec, p->fd, n, p->C, 0, n, st, p->static_plans != 0, p->slab_fused != 0)
                        : launch_fwd_zy(p->q, p->spec, p->fd, n, p->C, 0, n, st, 0, p->static_plans != 0,
                                        p->slab_fused != 0);
     if (rc != VX_OK) return rc;
@@ -2282,7 +2273,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
         VX_LAUNCH_CHECK();
     }
     VX_MARK(1);
-    rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->static_plans != 0)
+    rc = p->pencil ? launch_fwd_pencil(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->static_plans != 0, p->slab_fused != 0)
                    : launch_fwd_zy(ws.q, ws.spec, p->fd, n, nb * p->C, p->lo, p->w, st, p->slab_nt,
                                    p->static_plans != 0, p->slab_fused != 0);
     if (rc != VX_OK) return rc;
@@ -2306,7 +2297,7 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     // maps != nullptr: every group's real correlation map is kept (FP64
     // re-scoring picks its candidates there); prob mixing reads them too
     float* ro = maps ? maps : (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
-    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0)
+    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0, p->slab_fused != 0)
                    : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt,
                                    p->static_plans != 0, p->slab_fused != 0);
     if (rc != VX_OK) return rc;
@@ -2485,14 +2476,14 @@ extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float*
     const int xv = p->xvariant == 0 ? 0 : 1;   // the FFT x passes valid for every grid edge
     for (int c = 0; c < nch; ++c) {
         const float* qc = q + (size_t)c * n3;
-        int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st, p->static_plans != 0)
+        int rc = p->pencil ? launch_fwd_pencil(qc, p->spec, p->fd, n, 1, 0, n, st, p->static_plans != 0, p->slab_fused != 0)
                            : launch_fwd_zy(qc, p->spec, p->fd, n, 1, 0, n, st, 0, p->static_plans != 0,
                                            p->slab_fused != 0);
         if (rc != VX_OK) return rc;
         rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT,
                           p->static_plans != 0);
         if (rc != VX_OK) return rc;
-        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, p->static_plans != 0)
+        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, p->static_plans != 0, p->slab_fused != 0)
                        : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, 0,
                                        p->static_plans != 0, p->slab_fused != 0);
         if (rc != VX_OK) return rc;

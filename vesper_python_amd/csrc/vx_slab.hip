@@ -1,10 +1,11 @@
 // (y, z) slab kernels on compile-time FFT plans with their boundary passes
 // folded into the FFT stages next to them: the forward slab reading its first
-// stage from the query planes (k_fwd_zy_fused), and the inverse slab
-// reducing its last stage into the argmax (k_inv_yz_fused).  The
-// separate-pass forms and the runtime-plan kernels live in vx_search.hip;
-// these are the default on every grid edge with a compile-time plan
-// (VX_TUNE_SLAB_FUSED).
+// stage from the query planes (k_fwd_zy_fused), the inverse slab reducing its
+// last stage into the argmax (k_inv_yz_fused), and the same three folds in the
+// pencil passes of the larger grids (k_fwd_z_fused, k_axis_y_fused,
+// k_inv_z_fused).  The separate-pass forms and the runtime-plan kernels live
+// in vx_search.hip; these are the default on every grid edge with a
+// compile-time plan (VX_TUNE_SLAB_FUSED).
 // Their own translation unit, so that they compile beside the search
 // kernels.
 #include <hip/hip_runtime.h>
@@ -307,6 +308,251 @@ k_inv_yz_fused(const float2* __restrict__ sum, FftDesc fdg, int G, unsigned long
     }
 }
 
+// ---------------------------------------------------------------------------
+// Pencil passes (grids whose (y, z) slab exceeds LDS) on compile-time plans,
+// with the same folding of boundary passes into the stages next to them.
+// ---------------------------------------------------------------------------
+
+// Forward r2c along z for a chunk of P line pairs of one x-slab (the
+// k_fwd_z of vx_search.hip): the first stage reads its butterflies' two real
+// rows from the planes (consecutive threads on consecutive butterflies of a
+// line: coalesced, zero outside the box's z range without a load), the later
+// stages run in place with consecutive threads on consecutive lines, then
+// the unpack writes the rows' half spectra.
+template <class SP>
+__global__ void __launch_bounds__(256)
+k_fwd_z_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int lo, int w, int P) {
+    static_assert(SP::kStatic && SP::kNS >= 2, "the fused pencil needs a compile-time plan of two stages or more");
+    constexpr int n = SP::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int NS = SP::kNS;
+    constexpr int NT = 256;
+    extern __shared__ float2 lds[];
+    float2* A = lds;
+    const FftDesc fd = stage_twiddles(fdg, lds + P * n);
+    const int x = lo + blockIdx.y;
+    const size_t plane = blockIdx.z;
+    const int l0 = (lo >> 1) + blockIdx.x * P;
+    const int l1 = min(l0 + P, ((lo + w - 1) >> 1) + 1);
+    const int np = l1 - l0;
+    if (np <= 0) return;
+    const float* src = q + (plane * n + x) * (size_t)n * n;
+    {
+        constexpr int R = SP::kR[0], T = n / R;
+        const int nb = T * np;
+#pragma unroll 1
+        for (int b0 = 0; b0 < nb; b0 += NT) {
+            const int b = b0 + (int)threadIdx.x;
+            if (b < nb) {
+                const int ll = b / T, i = b - ll * T, l = l0 + ll;
+                const float* r0 = src + (size_t)(2 * l) * n;
+                const bool hb = 2 * l + 1 < n;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int z = i + j * T;
+                    float re = 0.f, im = 0.f;
+                    if (z >= lo && z < lo + w) {
+                        re = r0[z];
+                        if (hb) im = r0[n + z];
+                    }
+                    v[j] = make_float2(re, im);
+                }
+                dft<R>(v);
+                float2* O = A + ll * n + i * R;   // nothing reads A before the barrier below
+#pragma unroll
+                for (int j = 0; j < R; ++j) O[j] = v[j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS>(A, fd.twid, np, 1.0f / (float)np);
+    float2* out = spec + (plane * n + x) * (size_t)n * nhz;
+    for (int u = threadIdx.x; u < np * nhz; u += NT) {
+        const int ll = u / nhz, k = u - ll * nhz, l = l0 + ll;
+        const float2 zk = A[ll * n + k];
+        const float2 wv = A[ll * n + (k == 0 ? 0 : n - k)];
+        out[(size_t)(2 * l) * nhz + k] = make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
+        if (2 * l + 1 < n) out[(size_t)(2 * l + 1) * nhz + k] = make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x));
+    }
+}
+
+// In-place complex FFT along y of a full tile of NL kz-columns of one x-slab
+// (the k_axis_y of vx_search.hip on its compile-time tile plans): the first
+// stage reads its inputs from HBM (rows outside [ylo, ylo + yw) as zero;
+// conjugated for the inverse), the middle stages ping-pong in LDS, and the
+// last stage, whose butterflies write the rows they read, stores to HBM.
+// Workgroups reach the XCDs as in k_axis_y.
+template <class TL>
+__global__ void __launch_bounds__(256)
+k_axis_y_fused(float2* __restrict__ buf, FftDesc fdg, int inverse, int ylo, int yw, int xlo, int ntile, int nxs,
+               int total) {
+    using PL = typename TL::Plan;
+    static_assert(PL::kNS >= 2, "the fused y pass needs a compile-time plan of two stages or more");
+    constexpr int n = PL::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int NL = TL::kNL;
+    constexpr int NS = PL::kNS;
+    constexpr int NT = 256;
+    extern __shared__ float2 lds[];
+    const int per = (total + 7) >> 3;
+    const int m = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if ((int)(blockIdx.x >> 3) >= per || m >= total) return;
+    const int tile = m % ntile, rest = m / ntile;
+    const int xs = rest % nxs, plane = rest / nxs;
+    float2* A = lds;
+    float2* Bf = lds + n * NL;
+    const FftDesc fd = stage_twiddles(fdg, lds + 2 * n * NL);
+    float2* base = buf + ((size_t)plane * n + xlo + xs) * (size_t)n * nhz + tile * NL;
+    {
+        constexpr int R = PL::kR[0], T = n / R, NB = T * NL;
+#pragma unroll 1
+        for (int b0 = 0; b0 < NB; b0 += NT) {
+            const int b = b0 + (int)threadIdx.x;
+            if (NB % NT == 0 || b < NB) {
+                const int i = b / NL, col = b - i * NL;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int y = i + j * T;
+                    float2 a = make_float2(0.f, 0.f);
+                    if (y >= ylo && y < ylo + yw) a = base[(size_t)y * nhz + col];
+                    v[j] = inverse ? cconj(a) : a;
+                }
+                dft<R>(v);
+                float2* O = A + b + (R - 1) * NL * i;   // nothing reads A before the barrier below
+#pragma unroll
+                for (int j = 0; j < R; ++j) O[j * NL] = v[j];
+            }
+        }
+        __syncthreads();
+    }
+    const float2* r = tile_stages_pp<PL, NL, NT, 1, NS - 1>(A, Bf, fd.twid);
+    {
+        constexpr int S = NS - 1, R = PL::kR[S], P = PL::P(S), T = n / R, NB = T * NL;
+        static_assert(T == P, "the last stage of a plan spans its earlier radices");
+#pragma unroll 1
+        for (int b0 = 0; b0 < NB; b0 += NT) {
+            const int b = b0 + (int)threadIdx.x;
+            if (NB % NT == 0 || b < NB) {
+                const int i = b / NL, col = b - i * NL;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = r[b + j * T * NL];
+                twiddle_dft<R, P, PL::TW(S)>(v, i, fd.twid);
+#pragma unroll
+                for (int j = 0; j < R; ++j) base[(size_t)(i + j * P) * nhz + col] = inverse ? cconj(v[j]) : v[j];
+            }
+        }
+    }
+}
+
+// Inverse c2r along z for a chunk of P line pairs (the k_inv_z of
+// vx_search.hip): the Hermitian pack of rows (2l, 2l + 1) of the y-inverted
+// half spectrum happens inside the first stage's loads from HBM, the later
+// stages run in place with consecutive threads on consecutive lines, and the
+// last stage reduces its outputs into the argmax (and writes the real maps
+// when asked) from registers.
+template <class SP>
+__global__ void __launch_bounds__(256)
+k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, unsigned long long* __restrict__ keys,
+              int nkeys, float* __restrict__ real_out) {
+    static_assert(SP::kStatic && SP::kNS >= 2, "the fused pencil needs a compile-time plan of two stages or more");
+    constexpr int n = SP::kN;
+    constexpr int nhz = n / 2 + 1;
+    constexpr int npair = (n + 1) / 2;
+    constexpr int half = n / 2;
+    constexpr bool even = (n & 1) == 0;
+    constexpr int NS = SP::kNS;
+    constexpr int NT = 256;
+    extern __shared__ float2 lds[];
+    __shared__ float s_v[16];
+    __shared__ uint32_t s_i[16];
+    float2* A = lds;
+    const FftDesc fd = stage_twiddles(fdg, lds + P * n);
+    const int x = blockIdx.y;
+    const int bg = blockIdx.z;
+    const int l0 = blockIdx.x * P;
+    const int np = min(P, npair - l0);
+    const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
+    {
+        constexpr int R = SP::kR[0], T = n / R;
+        const int nb = T * np;
+#pragma unroll 1
+        for (int b0 = 0; b0 < nb; b0 += NT) {
+            const int b = b0 + (int)threadIdx.x;
+            if (b < nb) {
+                const int ll = b / T, i = b - ll * T, l = l0 + ll;
+                const bool hasb = (2 * l + 1) < n;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const int k = i + j * T;
+                    const int kk = (k <= half) ? k : n - k;
+                    float2 a = src[(size_t)(2 * l) * nhz + kk];
+                    float2 bb = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+                    if (k <= half) {
+                        if (k == 0 || (even && k == half)) { a.y = 0.f; bb.y = 0.f; }
+                        v[j] = make_float2(a.x - bb.y, -a.y - bb.x);       // conj(A + iB)
+                    } else {
+                        v[j] = make_float2(a.x + bb.y, a.y - bb.x);        // conj(conj A' + i conj B')
+                    }
+                }
+                dft<R>(v);
+                float2* O = A + ll * n + i * R;   // nothing reads A before the barrier below
+#pragma unroll
+                for (int j = 0; j < R; ++j) O[j] = v[j];
+            }
+        }
+        __syncthreads();
+    }
+    slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS - 1>(A, fd.twid, np, 1.0f / (float)np);
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    {
+        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R;
+        static_assert(T == P, "the last stage of a plan spans its earlier radices");
+        const int nb = T * np;
+        const size_t n3 = (size_t)n * n * n;
+        const bool maps = real_out != nullptr;
+        const float npinv = 1.0f / (float)np;
+#pragma unroll 1
+        for (int b0 = 0; b0 < nb; b0 += NT) {
+            const int b = b0 + (int)threadIdx.x;
+            if (b < nb) {
+                int ll, i;
+                if (maps) { ll = b / T; i = b - ll * T; }
+                else      { i = fdiv(b, np, npinv); ll = b - imul(i, np); }
+                const int l = l0 + ll;
+                float2 v[R];
+#pragma unroll
+                for (int j = 0; j < R; ++j) v[j] = A[ll * n + i + j * T];
+                twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
+                const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
+                const bool hasb = 2 * l + 1 < n;
+#pragma unroll
+                for (int j = 0; j < R; ++j) {
+                    const uint32_t ia = row + i + j * P;
+                    const float va = v[j].x;
+                    if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
+                    if (maps) real_out[(size_t)bg * n3 + ia] = va;
+                    if (hasb) {
+                        const uint32_t ib = ia + n;
+                        const float vb = -v[j].y;
+                        if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
+                        if (maps) real_out[(size_t)bg * n3 + ib] = vb;
+                    }
+                }
+            }
+        }
+    }
+    block_argmax(bv, bi, s_v, s_i);
+    if (threadIdx.x == 0) {
+        const int b = bg / G, g = bg - b * G;
+        atomicMax(&keys[(size_t)b * nkeys + g], argmax_key(bv, bi));
+    }
+}
+
 #define VX_FUSED_LAUNCH(KER, MV, NTH, SPT, ...)                                        \
     do {                                                                               \
         allow_lds(KER<MV, NTH, SPT>, lds);                                             \
@@ -379,6 +625,70 @@ bool launch_inv_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int 
             using SP = decltype(sp);
             VX_FUSED_LAUNCH(k_inv_yz_fused, 24, 1024, SP, sum, fd, G, keys, nkeys, real_out);
         });
+    if (hit) *rc = launch_status();
+    return hit;
+}
+
+// The fused pencil passes where the grid edge has a compile-time plan
+// (201..400); each returns false (nothing launched) otherwise.
+bool launch_fwd_z_fused(const float* q, float2* spec, const FftDesc& fd, int n, int nplanes, int lo, int w,
+                        hipStream_t st, int* rc) {
+    *rc = VX_OK;
+    const int P = pencil_pairs(n);
+    if (n * P > 256 * kPencilMaxV) return false;
+    const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
+    const dim3 gz((l1 - l0 + P - 1) / P, w, nplanes);
+    const size_t lds = ((size_t)P * n + fd.ntw) * sizeof(float2);
+    if (lds > kMaxLds) return false;
+    auto go = [&](auto sp) {
+        using SP = decltype(sp);
+        allow_lds(k_fwd_z_fused<SP>, lds);
+        hipLaunchKernelGGL(k_fwd_z_fused<SP>, gz, dim3(256), lds, st, q, spec, fd, lo, w, P);
+    };
+    const bool hit = with_static_plan(fd, TileLengths{}, go) || with_static_plan(fd, LargeLengths{}, go);
+    if (hit) *rc = launch_status();
+    return hit;
+}
+
+bool launch_axis_y_fused(float2* buf, const FftDesc& fd, int n, int inverse, int ylo, int yw, int xlo, int nxs,
+                         int nplanes, hipStream_t st, int* full_tiles, int* rc) {
+    *rc = VX_OK;
+    const int nhz = n / 2 + 1;
+    const int T = pencil_T(n);
+    const int full = nhz / T;
+    *full_tiles = full;
+    const size_t lds = (2 * (size_t)n * T + fd.ntw) * sizeof(float2);
+    if (full <= 0 || lds > kMaxLds) return false;
+    const int total = full * nxs * nplanes;
+    const dim3 grid((unsigned)(((total + 7) / 8) * 8));
+    auto go = [&](auto sp, auto width) {
+        using TL = typename decltype(sp)::template Tile<decltype(width)::value>;
+        allow_lds(k_axis_y_fused<TL>, lds);
+        hipLaunchKernelGGL(k_axis_y_fused<TL>, grid, dim3(256), lds, st, buf, fd, inverse, ylo, yw, xlo, full, nxs,
+                           total);
+    };
+    bool hit = false;
+    if (T == 8) hit = with_static_plan(fd, TileLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 8>{}); });
+    else if (T == 4) hit = with_static_plan(fd, LargeLengths{}, [&](auto sp) { go(sp, std::integral_constant<int, 4>{}); });
+    if (hit) *rc = launch_status();
+    return hit;
+}
+
+bool launch_inv_z_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
+                        int nkeys, float* real_out, hipStream_t st, int* rc) {
+    *rc = VX_OK;
+    const int P = pencil_pairs(n);
+    if (n * P > 256 * kPencilMaxV) return false;
+    const int npair = (n + 1) / 2;
+    const dim3 gz((npair + P - 1) / P, n, nbg);
+    const size_t lds = ((size_t)P * n + fd.ntw) * sizeof(float2);
+    if (lds > kMaxLds) return false;
+    auto go = [&](auto sp) {
+        using SP = decltype(sp);
+        allow_lds(k_inv_z_fused<SP>, lds);
+        hipLaunchKernelGGL(k_inv_z_fused<SP>, gz, dim3(256), lds, st, sum, fd, G, P, keys, nkeys, real_out);
+    };
+    const bool hit = with_static_plan(fd, TileLengths{}, go) || with_static_plan(fd, LargeLengths{}, go);
     if (hit) *rc = launch_status();
     return hit;
 }
