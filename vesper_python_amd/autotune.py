@@ -103,6 +103,10 @@ def _store(d: dict) -> None:
         pass
 
 
+# launches over which one and two batches in flight are compared
+STREAM_LAUNCHES = 32
+
+
 def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True, verbose: bool = False,
          mixed_precision: bool = True) -> dict:
     """Pick the fastest schedule for `plan` on rows `rot` (device [R, 9]);
@@ -203,8 +207,13 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
         if t2 < total:
             total, best, ms = t2, cand, ms2
     plan.tune(radix_order=best["radix_order"])
-    # two batches in flight: only a wall-clock measurement shows the overlap
+    # two batches in flight: only a wall-clock measurement shows the overlap,
+    # and only over enough launches (a 4-launch sample is mostly the fill
+    # and drain of the pipeline): the sample repeated to 32 launches; two
+    # streams unless one is more than 1 % faster
     if nrows >= 2 * plan.batch:
+        reps = max(1, (STREAM_LAUNCHES * plan.batch + nrows - 1) // nrows)
+        long_sample = sample.repeat(reps, 1)[:STREAM_LAUNCHES * plan.batch]
         walls = {}
         for ns in (1, 2):
             try:
@@ -214,17 +223,17 @@ def tune(plan, rot: torch.Tensor, sample_rows: int = 128, use_cache: bool = True
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-                for _ in range(3):
-                    plan.score_rows(sample)
+                for _ in range(2):
+                    plan.score_rows(long_sample)
                 e1.record()
                 torch.cuda.synchronize()
-                walls[ns] = e0.elapsed_time(e1) / 3
+                walls[ns] = e0.elapsed_time(e1) / 2
             except Exception as e:
                 if verbose:
                     print("[tune] skip streams", ns, e, file=sys.stderr)
         if verbose and walls:
-            print(f"[tune] streams wall ms {walls}", file=sys.stderr, flush=True)
-        if 2 in walls and 1 in walls and walls[2] < walls[1]:
+            print(f"[tune] streams wall ms over {STREAM_LAUNCHES} launches {walls}", file=sys.stderr, flush=True)
+        if 2 in walls and (1 not in walls or walls[1] >= 0.99 * walls[2]):
             best = dict(best, streams=2)
         else:
             best = dict(best, streams=1)
