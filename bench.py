@@ -133,6 +133,39 @@ def cpu_baseline(n1, n2, angles, budget_s: float, nrows_max: int):
                 **host_info(), _rows=list(range(0, done * stride, stride)), _scores=got)
 
 
+def cpu_stages(w, n1, n2, planes: int = 8):
+    """The steps beside the search loop on one host core, as SURVEY's CPU
+    column times them: fastVEC of the query map by the oracle's C
+    restatement of doFastVEC (exact order, float64) on `planes` output
+    x-planes spread over the grid, extrapolated to all of them; and the FP64
+    get_score at one translation by the oracle (NumPy).  The same calls on
+    the GPU are in aux_rooflines."""
+    from threadpoolctl import threadpool_limits
+    from oracle import build_c
+    from oracle import vesper_ref as ref
+    m1, m2 = w["src_maps"]
+    n = int(n2.xdim)
+    lib = build_c.load()
+    src = np.ascontiguousarray(np.asarray(m2.data, np.float32))
+    xs = [int((k + 0.5) * n / planes) for k in range(planes)]
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        for x in xs:
+            build_c.fastvec_c(lib, src, m2.xwidth, m2.orig, n, n2.xwidth, n2.orig, w["g"], planes=(x, x + 1))
+        t_fv = time.perf_counter() - t0
+        t = ref.MapGrid(data=np.asarray(n1.data, np.float64), xwidth=np.float32(n1.xwidth), orig=n1.orig,
+                        vec=n1.vec, cent=n1.cent, xdim=n1.xdim, ave=n1.ave, std=n1.std,
+                        std_norm_ave=n1.std_norm_ave)
+        t0 = time.perf_counter()
+        ref.get_score(t, np.asarray(n2.data, np.float64), n2.vec, (3, -2, 5), prec="ref64")
+        t_gs = time.perf_counter() - t0
+    return {"fastvec_query_map": {"seconds_est": round(t_fv / planes * n, 2), "planes_timed": planes, "n": n,
+                                  "what": "oracle C doFastVEC restatement, float64, 1 thread, "
+                                          f"{planes} of {n} output x-planes timed ({t_fv:.1f} s), scaled"},
+            "get_score_fp64": {"seconds": round(t_gs, 3), "what": "oracle NumPy get_score (ref64), 1 thread, "
+                                                                   "one translation"}}
+
+
 def host_info():
     """The host the CPU baseline ran on: CPU model and the CPUs this process
     may use."""
@@ -1043,6 +1076,15 @@ def main():
     if cpu:
         cpu["parity_vs_gpu"] = baseline_parity(cpu, best, trans)
         rec["gpu_vs_cpu"] = round(value / cpu["value"], 1)
+        try:
+            st = cpu_stages(w, n1, n2)
+            if aux:
+                st["fastvec_query_map"]["gpu_ms"] = aux["fastvec"]["ms"]
+                st["get_score_fp64"]["gpu_ms"] = aux["get_score"]["ms"]
+            st["search_row"] = {"seconds": round(1.0 / cpu["value"], 3), "gpu_ms": round(1e3 / value, 4)}
+            cpu["stages"] = st
+        except Exception as e:  # the search-loop figure stays the reported baseline
+            print(f"[bench] CPU stage timings skipped: {e}", file=sys.stderr, flush=True)
     if cpu_all:
         rec["cpu_baseline_all_cores"] = cpu_all
         rec["gpu_vs_cpu_all_cores"] = round(value / cpu_all["value"], 1)
