@@ -18,6 +18,11 @@
 // The channel sum happens in the frequency domain (linearity of the inverse
 // transform), so there is one inverse per group instead of one per channel
 // (VESPER_1_prob_new.py:464-491 + :357-374 compute C inverses then sum).
+// On the grid edges with compile-time FFT plans the slab kernels (and, above
+// N = 200, the pencil passes that replace them) run in the folded forms of
+// vx_slab.hip: first stage read from HBM, unpack / pack inside the next
+// stage, last stage stored or reduced into the argmax from registers.  The
+// kernels here are the separate-pass and runtime-plan forms.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
