@@ -48,6 +48,8 @@ def test_errors_are_status_codes_not_aborts():
     assert rc == -1 and b"mode" in lib.vx_last_error()
     rc = lib.vx_topn(None, 10, 1, 0, 5, None, None)
     assert rc == -1
+    rc = lib.vx_query_planes(None, None, None, None)
+    assert rc == -1 and b"NULL" in lib.vx_last_error()
     assert lib.vx_plan_destroy(None) == 0
 
 
