@@ -761,36 +761,58 @@ def measure_proposal(plan, n1, n2, ang, best, R, plain_rate):
                       "then neighbour sweeps around the best rows"}
 
 
-def _uneven_resampled(seed, s, g, tmp):
-    from vesper_python_amd import synth
+def _pairs_in_chunks(seeds, tmp, workers, chunk=32):
+    """Uneven-generator pairs written as MRC files by a pool of host
+    processes, yielded one at a time (seed, paths, thresholds); the files of
+    at most two chunks of pairs exist at once, and each pair's are removed
+    after use."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    chunks = [seeds[c0:c0 + chunk] for c0 in range(0, len(seeds), chunk)]
+    with ProcessPoolExecutor(max_workers=workers, mp_context=mp.get_context("spawn")) as ex:
+        fut = [ex.submit(_write_uneven_pair, (sd, tmp)) for sd in chunks[0]] if chunks else []
+        for ci in range(len(chunks)):
+            # the next chunk is written while this one is used
+            nxt = [ex.submit(_write_uneven_pair, (sd, tmp)) for sd in chunks[ci + 1]] if ci + 1 < len(chunks) else []
+            for f in fut:
+                item = f.result()
+                yield item
+                for path in item[1].values():
+                    if os.path.exists(path):
+                        os.remove(path)
+            fut = nxt
+
+
+def _resampled(item, s, g):
     from vesper_python_amd import VESPER_1_prob_new as V
     from vesper_python_amd import main as cli
-    pair = synth.make_uneven_pair(seed)
-    paths = synth.write_pair(pair, os.path.join(tmp, f"p{seed}"))
+    seed, paths, t1, t2 = item
     with contextlib.redirect_stdout(io.StringIO()):
-        m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], pair.a.threshold, pair.b.threshold, s, g)
+        m1, m2, n1, n2 = cli._resample_pair(paths["A"], paths["B"], t1, t2, s, g)
         n1 = V.fastVEC(m1, n1, g)
         n2 = V.fastVEC(m2, n2, g)
-    for f in paths.values():
-        os.remove(f)
     return n1, n2
 
 
 def measure_ablation(n_train=16, n_test=8, s=3.0, g=16.0, label_ang=10.0, judge_ang=5.0, coarse_ang=30.0,
-                     fracs=(0.0025, 0.005, 0.01, 0.05), seed_train=6000, seed_test=7000):
+                     fracs=(0.0025, 0.005, 0.01, 0.05), seed_train=6000, seed_test=7000, epochs=8,
+                     workers=16, weights_in=None, weights_out=None):
     """What the learned proposer gains over the same search without it (the
-    config-5 protocol, scaled): the network is pretrained on the exhaustive
-    -A label_ang landscapes of n_train pairs from the uneven generator, then
-    on each of n_test held-out pairs three policies spend the same budget --
-    the -A coarse_ang landscape, a fraction (each of `fracs`) of the
-    -A judge_ang rows in two shares, three neighbour sweeps around the best
-    rows -- choosing the shares by
-    (mlp) the network refined on the pair's coarse landscape, (coarse) the
-    coarse landscape's score of each row's nearest coarse row, (random)
-    uniform draws.  Judge: the exhaustive -A judge_ang search of the pair
-    (top-10 recall, and the gap between the best score found and the
-    exhaustive best).  Maps resampled at -s s (2x the config's 1.5 A, so that
-    the whole protocol fits a few minutes)."""
+    config-5 protocol; its full size is 512 training and 64 held-out pairs):
+    the network is pretrained on the exhaustive -A label_ang landscapes of
+    n_train pairs from the uneven generator (streamed: one pair's plan on the
+    device at a time), then on each of n_test held-out pairs three policies
+    spend the same budget -- the -A coarse_ang landscape, a fraction (each
+    of `fracs`) of the -A judge_ang rows in two shares, three neighbour
+    sweeps around the best rows -- choosing the shares by (mlp) the network
+    refined on the pair's coarse landscape, (coarse) the coarse landscape's
+    score of each row's nearest coarse row, (random) uniform draws.  Judge:
+    the exhaustive -A judge_ang search of the pair (top-10 recall, and the
+    gap between the best score found and the exhaustive best).  Maps
+    resampled at -s s (2x the config's 1.5 A, so that the protocol fits one
+    GPU call).  weights_out saves the pretrained network (.npz) and
+    weights_in loads one instead of pretraining, so that training and the
+    held-out evaluation can run separately."""
     import torch
     from vesper_python_amd import _native as nat
     from vesper_python_amd import engine, proposer
@@ -807,22 +829,36 @@ def measure_ablation(n_train=16, n_test=8, s=3.0, g=16.0, label_ang=10.0, judge_
         plan.set_search(*n2._dev, None, n1.ave)
         return proposer.Proposer(plan, *n1._dev, *n2._dev, mlp)
 
+    out = {}
     t0 = time.perf_counter()
-    train = []
-    for k in range(n_train):
-        n1, n2 = _uneven_resampled(seed_train + k, s, g, tmp)
-        train.append(prop(n1, n2, None))
-        print(f"[ablation] train pair {k + 1}/{n_train} N={int(n1.xdim)}", file=sys.stderr, flush=True)
-    mlp = proposer.RotationMLP(seed=21)
-    losses = proposer.pretrain(mlp, train, labels, epochs=8, lr=2e-3)
-    st = mlp.state()
-    for pr in train:
-        pr.plan.close()
-    del train
-    t_train = time.perf_counter() - t0
+    if weights_in:
+        with np.load(weights_in) as z:
+            st = {k: z[k] for k in z.files}
+        out["pretrained_from"] = os.path.basename(weights_in)
+    else:
+        it = _pairs_in_chunks([seed_train + k for k in range(n_train)], tmp, workers)
+        edges = []
+
+        def make(k):
+            return prop(*_resampled(next(it), s, g), None)
+
+        def progress(k, n):
+            edges.append(int(n))
+            if (k + 1) % 8 == 0 or k + 1 == n_train:
+                print(f"[ablation] train pair {k + 1}/{n_train} N={n}", file=sys.stderr, flush=True)
+        mlp = proposer.RotationMLP(seed=21)
+        losses = proposer.pretrain_streamed(mlp, make, n_train, labels, epochs=epochs, lr=2e-3, progress=progress)
+        st = mlp.state()
+        if weights_out:
+            np.savez(weights_out, **st)
+        out.update(pretrain_loss=[round(losses[0], 4), round(losses[-1], 4)],
+                   pretrain_s=round(time.perf_counter() - t0, 1), train_rows=len(labels) * n_train,
+                   train_grid_edges=[min(edges), max(edges)])
+        print(f"[ablation] pretrained on {n_train} pairs in {out['pretrain_s']} s, loss {out['pretrain_loss']}",
+              file=sys.stderr, flush=True)
     per = []
-    for k in range(n_test):
-        n1, n2 = _uneven_resampled(seed_test + k, s, g, tmp)
+    for k, item in enumerate(_pairs_in_chunks([seed_test + k for k in range(n_test)], tmp, workers)):
+        n1, n2 = _resampled(item, s, g)
         pr = prop(n1, n2, proposer.RotationMLP(seed=22))
         rot = torch.from_numpy(rotation_rows(fine)).cuda()
         torch.cuda.synchronize()
@@ -832,7 +868,7 @@ def measure_ablation(n_train=16, n_test=8, s=3.0, g=16.0, label_ang=10.0, judge_
         torch.cuda.synchronize()
         t_ex = time.perf_counter() - te
         b = best[:, 0].double().cpu().numpy()
-        rec = {"seed": seed_test + k, "grid_edge": int(n1.xdim), "exhaustive_s": round(t_ex, 3)}
+        rec = {"seed": item[0], "grid_edge": int(n1.xdim), "exhaustive_s": round(t_ex, 3)}
         for frac in fracs:
             for pol in proposer.POLICIES:
                 pr.mlp.load(st)
@@ -851,23 +887,23 @@ def measure_ablation(n_train=16, n_test=8, s=3.0, g=16.0, label_ang=10.0, judge_
         per.append(rec)
         pr.plan.close()
     table = {}
-    for frac in fracs:
+    for frac in (fracs if per else ()):
         row = {}
         for p in proposer.POLICIES:
             recs = [r[f"{p}@{frac}"] for r in per]
             row[p] = {"mean_recall_top10": round(float(np.mean([x["recall_top10"] for x in recs])), 4),
+                      "pairs_full_recall": int(sum(x["recall_top10"] == 1.0 for x in recs)),
                       "mean_best_gap": round(float(np.mean([x["best_gap"] for x in recs])), 6),
                       "mean_fft_rows": int(np.mean([x["fft_rows"] for x in recs]))}
         row["gain_over_coarse"] = round(row["mlp"]["mean_recall_top10"] - row["coarse"]["mean_recall_top10"], 4)
         row["gain_over_random"] = round(row["mlp"]["mean_recall_top10"] - row["random"]["mean_recall_top10"], 4)
         table[str(frac)] = row
-    return {"protocol": f"pretrain on {n_train} uneven pairs (exhaustive -A {label_ang} labels), judge {n_test} "
+    out.update(protocol=f"pretrain on {n_train} uneven pairs (exhaustive -A {label_ang} labels), judge {n_test} "
                         f"held-out pairs by exhaustive -A {judge_ang} ({len(fine)} rows); budget per policy: "
                         f"-A {coarse_ang} landscape + the fraction of the rows in 2 shares + 3 neighbour sweeps; "
                         f"-s {s} -g {g}",
-            "by_fraction": table,
-            "pretrain_loss": [round(losses[0], 4), round(losses[-1], 4)], "pretrain_s": round(t_train, 1),
-            "pairs": per}
+               by_fraction=table, pairs=per)
+    return out
 
 
 def main():
@@ -894,6 +930,13 @@ def main():
                     help="only the 48-pair uneven workload at -s 1.5 -A 10; prints its own JSON line")
     ap.add_argument("--ablation", action="store_true",
                     help="only the learned-proposer ablation (held-out pairs, -A 5 judge); prints its own JSON line")
+    ap.add_argument("--ablation-pairs", default="16,8",
+                    help="training and held-out pairs of the ablation (the protocol's full size: 512,64)")
+    ap.add_argument("--ablation-fracs", default="0.0025,0.005,0.01,0.05",
+                    help="shares of the -A 5 rows each policy may FFT-score")
+    ap.add_argument("--ablation-epochs", type=int, default=8)
+    ap.add_argument("--ablation-weights-out", default="", help="save the pretrained network (.npz)")
+    ap.add_argument("--ablation-weights-in", default="", help="load a pretrained network instead of training")
     ap.add_argument("--aux", action="store_true",
                     help="only the rooflines of fastVEC, get_score and top-N on the workload; prints its own JSON line")
     ap.add_argument("--schedule", default="", help='fixed kernel schedule instead of autotuning, as JSON, '
@@ -911,7 +954,11 @@ def main():
 
     engine.require_gpu()
     if args.ablation:
-        print(json.dumps({"ablation": measure_ablation()}), flush=True)
+        ntr, nte = (int(v) for v in args.ablation_pairs.split(","))
+        print(json.dumps({"ablation": measure_ablation(
+            n_train=ntr, n_test=nte, fracs=tuple(float(v) for v in args.ablation_fracs.split(",")),
+            epochs=args.ablation_epochs, weights_in=args.ablation_weights_in or None,
+            weights_out=args.ablation_weights_out or None)}), flush=True)
         return
     if args.pairs48:
         print(json.dumps({"pairs48": measure_pairs48()}), flush=True)

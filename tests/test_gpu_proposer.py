@@ -122,6 +122,15 @@ def test_training_is_repeatable(gpu):
     assert runs[0][0] == runs[1][0]
     for a, b in zip(runs[0][1], runs[1][1]):
         assert torch.equal(a, b)
+    print("[case] fit after load ignores earlier fits", flush=True)
+    # load() restarts the optimiser: the fit after it is the fresh network's
+    m = proposer.RotationMLP(seed=3)
+    st = m.state()
+    m.fit(X, yd, M, epochs=2, batch=256, lr=2e-3, seed=5)
+    m.load(st)
+    assert m.fit(X, yd, M, epochs=4, batch=256, lr=2e-3, seed=1) == runs[0][0]
+    for a, b in zip(m.params, runs[0][1]):
+        assert torch.equal(a.cpu(), b)
     n = 24
     data = rng.uniform(0, 1, (n, n, n)) * (rng.uniform(size=(n, n, n)) < 0.3)
     vec = rng.standard_normal((n, n, n, 3)).astype(np.float32) * (data > 0)[..., None]

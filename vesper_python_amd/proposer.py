@@ -191,10 +191,17 @@ class RotationMLP:
         return {f"p{i}": p.cpu().numpy() for i, p in enumerate(self.params)}
 
     def load(self, st: dict):
+        """Parameters from `st`; the optimiser starts afresh (Adam moments
+        zero, step count 0), so a fit after load does not depend on what the
+        network was fitted to before."""
         for i, p in enumerate(self.params):
             p.copy_(torch.from_numpy(st[f"p{i}"]))
         for w, wb in zip(self.W, self.Wb):
             nat.call("vx_cast_bf16", w.numel(), _p(w), _p(wb), _st())
+        for a, b in zip(self.m1, self.m2):
+            a.zero_()
+            b.zero_()
+        self.steps = 0
 
 
 def landscape(plan: engine.SearchPlan, rot: torch.Tensor) -> torch.Tensor:
@@ -265,6 +272,31 @@ def pretrain(mlp: RotationMLP, pairs, angles, epochs=20, lr=2e-3, batch=2048, se
     return mlp.fit(Xp, y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
 
 
+def pretrain_streamed(mlp: RotationMLP, make_proposer, n_pairs: int, angles, epochs=20, lr=2e-3, batch=2048,
+                      seed=0, progress=None):
+    """pretrain over n_pairs pairs made one at a time: make_proposer(k)
+    returns the k-th pair's Proposer, whose features and z-scored exhaustive
+    landscape go into the training set before its plan is closed, so that one
+    plan at a time holds device memory (the 512-pair training set of the
+    protocol is 12.6 M rows).  progress(k, grid_edge) after each pair.
+    Returns the per-epoch losses."""
+    R = len(angles)
+    M = R * n_pairs
+    dev = mlp.dev
+    Xp = torch.zeros((pad64(M), IN), dtype=torch.bfloat16, device=dev)
+    y = torch.empty(M, dtype=torch.float32, device=dev)
+    stats = torch.empty(2, dtype=torch.float64, device=dev)
+    for k in range(n_pairs):
+        pr = make_proposer(k)
+        rot, _ = pr.rows_features(angles, out=Xp[k * R:])
+        zscore_into(landscape(pr.plan, rot), R, y[k * R:], stats)
+        n = pr.plan.n
+        pr.plan.close()
+        if progress:
+            progress(k, n)
+    return mlp.fit(Xp, y, M, epochs=epochs, batch=batch, lr=lr, seed=seed)
+
+
 def grid_shape(angles):
     """(nx, nz) when `angles` is a calc_angle_comb grid (row = iz nx^2 + ix nx + iy
     for angle (x[ix], x[iy], z[iz])), else None."""
@@ -313,7 +345,7 @@ def nearest_coarse_rows(angles, coarse_step):
     return iz * nx * nx + ix * nx + iy, np.sqrt((d * d).sum(axis=1))
 
 
-POLICIES = ("mlp", "coarse", "random")
+POLICIES = ("mlp", "coarse", "random", "hybrid")
 
 
 def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, topn=10, epochs=40, lr=2e-3,
@@ -327,7 +359,11 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
     ablation with the same budget and sweeps, "coarse" (each fine row ranked
     by the coarse landscape's score of its nearest coarse row, nearer rows
     first; coarse_angles must then be the calc_angle_comb grid of step
-    `coarse_step`) or "random" (rows drawn uniformly).
+    `coarse_step`) or "random" (rows drawn uniformly); "hybrid" spends half of
+    each share on the coarse ranking (scored first, so the network retrains
+    on them) and the other half on the network's predictions: the coarse
+    landscape's best basins are always explored, and the network adds the
+    rows it ranks high elsewhere.
     Returns (top rows into angles, number of fine rows FFT-scored, coarse rows)."""
     if policy not in POLICIES:
         raise ValueError(f"unknown policy {policy!r}")
@@ -353,10 +389,13 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
     nscored = 0
     share = max(topn, int(math.ceil(frac * R / rounds)))
     static_pred = None
-    if policy == "coarse":
+    coarse_pred = None
+    if policy in ("coarse", "hybrid"):
         near, dist = nearest_coarse_rows(angles, coarse_step)
         base = trainy[:Rc].cpu().numpy().astype(np.float64)[near]
-        static_pred = torch.from_numpy((base - 1e-6 * dist).astype(np.float32)).to(dev)
+        coarse_pred = torch.from_numpy((base - 1e-6 * dist).astype(np.float32)).to(dev)
+        if policy == "coarse":
+            static_pred = coarse_pred
     elif policy == "random":
         static_pred = torch.from_numpy(np.random.default_rng(seed).random(R).astype(np.float32)).to(dev)
     cnt = np.zeros(1, dtype=np.int32)
@@ -369,26 +408,42 @@ def propose_rounds(pr: "Proposer", coarse_angles, angles, frac=0.05, rounds=2, t
         nat.call("vx_scatter_scores", n, _p(sel), _p(b), int(b.shape[1]), _p(best_all), _p(scored), _st())
         return b
 
-    for r in range(rounds):
-        if static_pred is None:
-            pr.mlp.fit(trainX, trainy, ntrain, epochs=epochs if r == 0 else max(5, epochs // 4), batch=512, lr=lr,
-                       seed=seed + r)
-            pred = pr.mlp.forward(X, R)
-        else:
-            pred = pred_buf.copy_(static_pred)
-        # rows already scored drop just below the lowest unscored prediction
+    def take(pred, k):
+        """FFT-score the unscored rows ranked highest by pred (in place: rows
+        already scored drop just below the lowest unscored prediction) -- the
+        top k and the rest of their histogram bin -- and add them to the
+        training set; returns how many were scored."""
+        nonlocal nscored, ntrain
         nat.call("vx_mask_scored", R, _p(pred), _p(scored), _st())
-        nat.call("vx_select_top", R, _p(pred), share, _p(rows), cnt_p, _st())
+        nat.call("vx_select_top", R, _p(pred), k, _p(rows), cnt_p, _st())
         nat.call("vx_filter_unscored", int(cnt[0]), _p(rows), _p(scored), _p(sel), cnt_p, _st())
         n = int(cnt[0])
         if n == 0:
-            break
+            return 0
         b = score_new(n)
         nscored += n
         gather_rows(X, sel, n, trainX[ntrain:])
         # labels on the coarse landscape's scale
         zscore_into(b, n, trainy[ntrain:], cstats, given=True)
         ntrain += n
+        return n
+
+    for r in range(rounds):
+        taken = 0
+        if policy == "hybrid":
+            # the coarse half (its threshold bin may overshoot; the network
+            # gets what is left of the share, possibly nothing)
+            taken = take(pred_buf.copy_(coarse_pred), (share + 1) // 2)
+        if static_pred is None:
+            pr.mlp.fit(trainX, trainy, ntrain, epochs=epochs if r == 0 else max(5, epochs // 4), batch=512, lr=lr,
+                       seed=seed + r)
+            pred = pr.mlp.forward(X, R)
+        else:
+            pred = pred_buf.copy_(static_pred)
+        if share - taken > 0:
+            taken += take(pred, share - taken)
+        if taken == 0:
+            break
     shape = grid_shape(angles) if sweeps else None
     for _ in range(sweeps if shape else 0):
         top = engine.topn(best_all, topn)
