@@ -658,6 +658,45 @@ def test_fused_slabs_identical(gpu, n, mode):
     assert np.array_equal(res[0][1], res[1][1]), (n, mode)
 
 
+@pytest.mark.parametrize("n,mode", [(208, nat.VX_MODE_VEC), (308, nat.VX_MODE_PROB), (343, nat.VX_MODE_CC)])
+def test_pencil_chunks_identical(gpu, n, mode):
+    """The fused inverse z pass walks several chunks of line pairs per
+    workgroup (the next chunk's rows load while the current one
+    transforms); every chunk count gives bit-identical scores and
+    translations, also where prob mixing reads the real maps it writes."""
+    rng = np.random.default_rng(9300 + n)
+    c = n / 2
+    g = np.indices((n, n, n), dtype=np.float32).transpose(1, 2, 3, 0) - c
+    inner = np.linalg.norm(g, axis=-1) <= 0.2 * n
+    del g
+    td, tv = _maps(rng, n, 0.02)
+    sd, sv = _maps(rng, n, 0.3)
+    sd[~inner] = 0
+    sv[~inner] = 0
+    angles = np.array([[0, 0, 0], [30, 60, 90], [200, 10, 170]], np.float64)
+    rot = torch.from_numpy(rotation_rows(angles)).cuda()
+    prob = None
+    if mode == nat.VX_MODE_PROB:
+        tp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (td != 0)
+        sp = rng.uniform(0.0, 1.0, (4, n, n, n)).astype(np.float32) * (sd != 0)
+        prob = (_dev(tp, torch.float32), _dev(sp, torch.float32))
+    plan = engine.SearchPlan(n, mode, batch=3)
+    plan.set_target(_dev(td, torch.float64), _dev(tv, torch.float32), prob[0] if prob else None, 0.3)
+    plan.set_search(_dev(sd, torch.float64), _dev(sv, torch.float32), prob[1] if prob else None, 0.3)
+    if prob:
+        plan.set_prob_stats(0.4, 2.0, 0.1, 3.0, 0.2, enable=True)
+    res = {}
+    for chunks in (1, 2, 3, 0, 1000):
+        _say("pencil chunks", n, mode, chunks)
+        plan.tune(pencil_chunks=chunks)
+        b, t = plan.score_rows(rot)
+        res[chunks] = (b.cpu().numpy(), t.cpu().numpy())
+    plan.close()
+    for chunks, (b, t) in res.items():
+        assert np.array_equal(b, res[1][0]), (n, mode, chunks)
+        assert np.array_equal(t, res[1][1]), (n, mode, chunks)
+
+
 @pytest.mark.parametrize("n,mode", [(35, nat.VX_MODE_VEC), (24, nat.VX_MODE_CC)])
 def test_xpass_variants_identical(gpu, n, mode):
     """The FFT x-pass kernels (register, LDS, register-prefetch; several

@@ -25,6 +25,7 @@ VX_TUNE_STREAMS = 7
 VX_TUNE_STATIC_PLANS = 8
 VX_TUNE_RADIX_ORDER = 9
 VX_TUNE_SLAB_FUSED = 10
+VX_TUNE_PENCIL_CHUNKS = 11
 ABI_VERSION = 1
 
 MODE_CODES = {"VecProduct": VX_MODE_VEC, "Overlap": VX_MODE_OVERLAP, "CC": VX_MODE_CC,

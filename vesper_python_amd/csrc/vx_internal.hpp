@@ -64,8 +64,9 @@ bool launch_fwd_z_fused(const float* q, float2* spec, const FftDesc& fd, int n, 
                         hipStream_t st, int* rc);
 bool launch_axis_y_fused(float2* buf, const FftDesc& fd, int n, int inverse, int ylo, int yw, int xlo, int nxs,
                          int nplanes, hipStream_t st, int* full_tiles, int* rc);
+// chunks: line-pair chunks per workgroup of the inverse z pass (0 = automatic)
 bool launch_inv_z_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
-                        int nkeys, float* real_out, hipStream_t st, int* rc);
+                        int nkeys, float* real_out, hipStream_t st, int* rc, int chunks);
 
 // Top-N selection (vx_score.hip): keys per ping-pong buffer, argument check,
 // and the selection on caller-provided buffers of topn_cap keys each.

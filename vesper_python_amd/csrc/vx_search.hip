@@ -1414,11 +1414,12 @@ int launch_fwd_pencil(const float* q, float2* spec, const FftDesc& fd, int n, in
 }
 
 int launch_inv_pencil(float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys, int nkeys,
-                      float* real_out, hipStream_t st, bool use_static = true, bool fused = true) {
+                      float* real_out, hipStream_t st, bool use_static = true, bool fused = true, int chunks = 0) {
     const int nhz = n / 2 + 1;
     int rc = launch_axis_y(sum, fd, n, 1, 0, n, 0, n, nbg, use_static, st, fused);
     if (rc != VX_OK) return rc;
-    if (use_static && fused && launch_inv_z_fused(sum, fd, n, nbg, G, keys, nkeys, real_out, st, &rc)) return rc;
+    if (use_static && fused && launch_inv_z_fused(sum, fd, n, nbg, G, keys, nkeys, real_out, st, &rc, chunks))
+        return rc;
     const int P = pencil_pairs(n);
     const size_t lz = (2 * (size_t)P * n + fd.ntw) * sizeof(float2);
     VX_REQUIRE(lz <= kMaxLds, VX_ERR_UNSUPPORTED, "grid edge %d too large", n);
@@ -1813,6 +1814,7 @@ struct vx_plan {
     int nstreams;     // 2: alternate batches over two workspaces and streams
     int static_plans; // 1: compile-time stage plans for the grid edges that have one
     int slab_fused;   // 1: fused slab kernels on compile-time plans (k_fwd_zy_fused, k_inv_yz_fused)
+    int pencil_chunks; // line-pair chunks per workgroup of the fused inverse z pass (0 = automatic)
     float* q2;        // second workspace (nstreams == 2)
     double* qd2;
     float2* spec2;
@@ -1887,6 +1889,7 @@ extern "C" int vx_plan_create(int n, int mode, int batch, int flags, vx_plan** o
     p->nstreams = 1;
     p->static_plans = 1;
     p->slab_fused = 1;
+    p->pencil_chunks = 0;
     switch (mode) {
         case VX_MODE_VEC:  p->C = 3; p->G = 1; p->nkeys = 1; p->kind = KIND_VEC; break;
         case VX_MODE_PROB: p->C = 7; p->G = 2; p->nkeys = 3; p->kind = KIND_PROB; break;
@@ -2202,6 +2205,10 @@ extern "C" int vx_plan_tune(vx_plan* p, int key, int value) {
             VX_REQUIRE(value == 0 || value == 1, VX_ERR_ARG, "slab fusion must be 0 or 1");
             p->slab_fused = value;
             break;
+        case VX_TUNE_PENCIL_CHUNKS:
+            VX_REQUIRE(value >= 0, VX_ERR_ARG, "pencil chunks must be >= 0");
+            p->pencil_chunks = value;
+            break;
         case VX_TUNE_SLAB_THREADS:
             VX_REQUIRE(value == 0 || value == 512 || value == 1024, VX_ERR_ARG,
                        "slab threads must be 0, 512 or 1024");
@@ -2302,7 +2309,8 @@ static int score_batch(vx_plan* p, const Workspace& ws, const float* rot, int r0
     // maps != nullptr: every group's real correlation map is kept (FP64
     // re-scoring picks its candidates there); prob mixing reads them too
     float* ro = maps ? maps : (p->mode == VX_MODE_PROB && p->mix) ? ws.real_out : nullptr;
-    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0, p->slab_fused != 0)
+    rc = p->pencil ? launch_inv_pencil(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->static_plans != 0, p->slab_fused != 0,
+                                       p->pencil_chunks)
                    : launch_inv_yz(ws.sum, p->fd, n, nb * p->G, p->G, ws.keys, nk, ro, st, p->slab_nt,
                                    p->static_plans != 0, p->slab_fused != 0);
     if (rc != VX_OK) return rc;
@@ -2488,7 +2496,8 @@ extern "C" int vx_correlate(vx_plan* p, const void* tspec, int nch, const float*
         rc = launch_xcorr(p->spec, ts + (size_t)c * nh, p->sum, p->fd, n, 1, 1, 1, 0u, 0, n, st, xv, p->xT,
                           p->static_plans != 0);
         if (rc != VX_OK) return rc;
-        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, p->static_plans != 0, p->slab_fused != 0)
+        rc = p->pencil ? launch_inv_pencil(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, p->static_plans != 0,
+                                           p->slab_fused != 0, p->pencil_chunks)
                        : launch_inv_yz(p->sum, p->fd, n, 1, 1, p->keys, 1, out + (size_t)c * n3, st, 0,
                                        p->static_plans != 0, p->slab_fused != 0);
         if (rc != VX_OK) return rc;

@@ -447,15 +447,19 @@ k_axis_y_fused(float2* __restrict__ buf, FftDesc fdg, int inverse, int ylo, int 
     }
 }
 
-// Inverse c2r along z for a chunk of P line pairs (the k_inv_z of
-// vx_search.hip): the Hermitian pack of rows (2l, 2l + 1) of the y-inverted
-// half spectrum happens inside the first stage's loads from HBM, the later
-// stages run in place with consecutive threads on consecutive lines, and the
-// last stage reduces its outputs into the argmax (and writes the real maps
-// when asked) from registers.
+// Inverse c2r along z over chunks of P line pairs of one x-slab (the k_inv_z
+// of vx_search.hip): the Hermitian pack of rows (2l, 2l + 1) of the
+// y-inverted half spectrum happens in the first stage, the later stages run
+// in place with consecutive threads on consecutive lines, and the last stage
+// reduces its outputs into the argmax (and writes the real maps when asked)
+// from registers.  Each workgroup walks K chunks [blockIdx.x K, ...): the raw
+// rows of the next chunk load into registers while the current chunk's
+// stages run, the twiddles are staged once, and one atomic publishes the
+// workgroup's argmax (the same maximum as one workgroup per chunk: the key
+// order is total).
 template <class SP>
 __global__ void __launch_bounds__(256)
-k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, unsigned long long* __restrict__ keys,
+k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, unsigned long long* __restrict__ keys,
               int nkeys, float* __restrict__ real_out) {
     static_assert(SP::kStatic && SP::kNS >= 2, "the fused pencil needs a compile-time plan of two stages or more");
     constexpr int n = SP::kN;
@@ -465,6 +469,10 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, unsigne
     constexpr bool even = (n & 1) == 0;
     constexpr int NS = SP::kNS;
     constexpr int NT = 256;
+    constexpr int R0 = SP::kR[0], T0 = n / R0;
+    // first-stage butterflies per thread: a chunk holds at most
+    // kPencilMaxV NT / n line pairs (the launcher's bound on P)
+    constexpr int RND = (T0 * (kPencilMaxV * NT / n) + NT - 1) / NT;
     extern __shared__ float2 lds[];
     __shared__ float s_v[16];
     __shared__ uint32_t s_i[16];
@@ -472,79 +480,102 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, unsigne
     const FftDesc fd = stage_twiddles(fdg, lds + P * n);
     const int x = blockIdx.y;
     const int bg = blockIdx.z;
-    const int l0 = blockIdx.x * P;
-    const int np = min(P, npair - l0);
+    const int c0 = blockIdx.x * K;
+    const int c1 = min(c0 + K, (npair + P - 1) / P);
     const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
-    {
-        constexpr int R = SP::kR[0], T = n / R;
-        const int nb = T * np;
-#pragma unroll 1
-        for (int b0 = 0; b0 < nb; b0 += NT) {
-            const int b = b0 + (int)threadIdx.x;
-            if (b < nb) {
-                const int ll = b / T, i = b - ll * T, l = l0 + ll;
-                const bool hasb = (2 * l + 1) < n;
-                float2 v[R];
+    // rows 2l and 2l + 1 at each first-stage input's half-spectrum index
+    float2 ra[RND][R0], rb[RND][R0];
+    auto load = [&](int c) {
+        const int l0 = c * P, nb = T0 * min(P, npair - l0);
 #pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const int k = i + j * T;
+        for (int r = 0; r < RND; ++r) {
+            const int b = r * NT + (int)threadIdx.x;
+            if (b < nb) {
+                const int ll = b / T0, i = b - ll * T0, l = l0 + ll;
+                const bool hasb = (2 * l + 1) < n;
+#pragma unroll
+                for (int j = 0; j < R0; ++j) {
+                    const int k = i + j * T0;
                     const int kk = (k <= half) ? k : n - k;
-                    float2 a = src[(size_t)(2 * l) * nhz + kk];
-                    float2 bb = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
-                    if (k <= half) {
-                        if (k == 0 || (even && k == half)) { a.y = 0.f; bb.y = 0.f; }
-                        v[j] = make_float2(a.x - bb.y, -a.y - bb.x);       // conj(A + iB)
-                    } else {
-                        v[j] = make_float2(a.x + bb.y, a.y - bb.x);        // conj(conj A' + i conj B')
+                    ra[r][j] = src[(size_t)(2 * l) * nhz + kk];
+                    rb[r][j] = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+                }
+            }
+        }
+    };
+    float bv = -INFINITY;
+    uint32_t bi = 0xffffffffu;
+    const size_t n3 = (size_t)n * n * n;
+    const bool maps = real_out != nullptr;
+    load(c0);
+#pragma unroll 1
+    for (int c = c0; c < c1; ++c) {
+        const int l0 = c * P, np = min(P, npair - l0);
+        {
+            const int nb = T0 * np;
+#pragma unroll
+            for (int r = 0; r < RND; ++r) {
+                const int b = r * NT + (int)threadIdx.x;
+                if (b < nb) {
+                    const int ll = b / T0, i = b - ll * T0;
+                    float2 v[R0];
+#pragma unroll
+                    for (int j = 0; j < R0; ++j) {
+                        const int k = i + j * T0;
+                        float2 a = ra[r][j], bb = rb[r][j];
+                        if (k <= half) {
+                            if (k == 0 || (even && k == half)) { a.y = 0.f; bb.y = 0.f; }
+                            v[j] = make_float2(a.x - bb.y, -a.y - bb.x);       // conj(A + iB)
+                        } else {
+                            v[j] = make_float2(a.x + bb.y, a.y - bb.x);        // conj(conj A' + i conj B')
+                        }
+                    }
+                    dft<R0>(v);
+                    float2* O = A + ll * n + i * R0;   // the previous chunk's reads of A ended at a barrier
+#pragma unroll
+                    for (int j = 0; j < R0; ++j) O[j] = v[j];
+                }
+            }
+        }
+        if (c + 1 < c1) load(c + 1);
+        __syncthreads();
+        slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS - 1>(A, fd.twid, np, 1.0f / (float)np);
+        {
+            constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R;
+            static_assert(T == P, "the last stage of a plan spans its earlier radices");
+            const int nb = T * np;
+            const float npinv = 1.0f / (float)np;
+#pragma unroll 1
+            for (int b0 = 0; b0 < nb; b0 += NT) {
+                const int b = b0 + (int)threadIdx.x;
+                if (b < nb) {
+                    int ll, i;
+                    if (maps) { ll = b / T; i = b - ll * T; }
+                    else      { i = fdiv(b, np, npinv); ll = b - imul(i, np); }
+                    const int l = l0 + ll;
+                    float2 v[R];
+#pragma unroll
+                    for (int j = 0; j < R; ++j) v[j] = A[ll * n + i + j * T];
+                    twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
+                    const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
+                    const bool hasb = 2 * l + 1 < n;
+#pragma unroll
+                    for (int j = 0; j < R; ++j) {
+                        const uint32_t ia = row + i + j * P;
+                        const float va = v[j].x;
+                        if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
+                        if (maps) real_out[(size_t)bg * n3 + ia] = va;
+                        if (hasb) {
+                            const uint32_t ib = ia + n;
+                            const float vb = -v[j].y;
+                            if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
+                            if (maps) real_out[(size_t)bg * n3 + ib] = vb;
+                        }
                     }
                 }
-                dft<R>(v);
-                float2* O = A + ll * n + i * R;   // nothing reads A before the barrier below
-#pragma unroll
-                for (int j = 0; j < R; ++j) O[j] = v[j];
             }
         }
         __syncthreads();
-    }
-    slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS - 1>(A, fd.twid, np, 1.0f / (float)np);
-    float bv = -INFINITY;
-    uint32_t bi = 0xffffffffu;
-    {
-        constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R;
-        static_assert(T == P, "the last stage of a plan spans its earlier radices");
-        const int nb = T * np;
-        const size_t n3 = (size_t)n * n * n;
-        const bool maps = real_out != nullptr;
-        const float npinv = 1.0f / (float)np;
-#pragma unroll 1
-        for (int b0 = 0; b0 < nb; b0 += NT) {
-            const int b = b0 + (int)threadIdx.x;
-            if (b < nb) {
-                int ll, i;
-                if (maps) { ll = b / T; i = b - ll * T; }
-                else      { i = fdiv(b, np, npinv); ll = b - imul(i, np); }
-                const int l = l0 + ll;
-                float2 v[R];
-#pragma unroll
-                for (int j = 0; j < R; ++j) v[j] = A[ll * n + i + j * T];
-                twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
-                const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
-                const bool hasb = 2 * l + 1 < n;
-#pragma unroll
-                for (int j = 0; j < R; ++j) {
-                    const uint32_t ia = row + i + j * P;
-                    const float va = v[j].x;
-                    if (better(va, ia, bv, bi)) { bv = va; bi = ia; }
-                    if (maps) real_out[(size_t)bg * n3 + ia] = va;
-                    if (hasb) {
-                        const uint32_t ib = ia + n;
-                        const float vb = -v[j].y;
-                        if (better(vb, ib, bv, bi)) { bv = vb; bi = ib; }
-                        if (maps) real_out[(size_t)bg * n3 + ib] = vb;
-                    }
-                }
-            }
-        }
     }
     block_argmax(bv, bi, s_v, s_i);
     if (threadIdx.x == 0) {
@@ -674,19 +705,34 @@ bool launch_axis_y_fused(float2* buf, const FftDesc& fd, int n, int inverse, int
     return hit;
 }
 
+// Chunks of line pairs per workgroup of the inverse z pass, of nch per
+// x-slab: `chunks` when positive, else equal shares that leave about 8192
+// workgroups in the launch (32 per CU; one atomic and one twiddle fill per
+// workgroup instead of one per chunk).
+static int pencil_chunks(int nch, int n, int nbg, int chunks) {
+    if (chunks <= 0) {
+        const long long per = (long long)n * nbg;   // workgroups per chunk share
+        const int shares = (int)std::min<long long>(nch, std::max<long long>(1, (8192 + per - 1) / per));
+        chunks = (nch + shares - 1) / shares;
+    }
+    return chunks < 1 ? 1 : (chunks > nch ? nch : chunks);
+}
+
 bool launch_inv_z_fused(const float2* sum, const FftDesc& fd, int n, int nbg, int G, unsigned long long* keys,
-                        int nkeys, float* real_out, hipStream_t st, int* rc) {
+                        int nkeys, float* real_out, hipStream_t st, int* rc, int chunks) {
     *rc = VX_OK;
     const int P = pencil_pairs(n);
     if (n * P > 256 * kPencilMaxV) return false;
     const int npair = (n + 1) / 2;
-    const dim3 gz((npair + P - 1) / P, n, nbg);
+    const int nch = (npair + P - 1) / P;
+    const int K = pencil_chunks(nch, n, nbg, chunks);
+    const dim3 gz((nch + K - 1) / K, n, nbg);
     const size_t lds = ((size_t)P * n + fd.ntw) * sizeof(float2);
     if (lds > kMaxLds) return false;
     auto go = [&](auto sp) {
         using SP = decltype(sp);
         allow_lds(k_inv_z_fused<SP>, lds);
-        hipLaunchKernelGGL(k_inv_z_fused<SP>, gz, dim3(256), lds, st, sum, fd, G, P, keys, nkeys, real_out);
+        hipLaunchKernelGGL(k_inv_z_fused<SP>, gz, dim3(256), lds, st, sum, fd, G, P, K, keys, nkeys, real_out);
     };
     const bool hit = with_static_plan(fd, TileLengths{}, go) || with_static_plan(fd, LargeLengths{}, go);
     if (hit) *rc = launch_status();

@@ -210,11 +210,12 @@ class SearchPlan:
     def tune(self, xpass_variant: int | None = None, xpass_tile: int | None = None,
              pencil: bool | None = None, slab_threads: int | None = None, streams: int | None = None,
              static_plans: bool | None = None, radix_order: int | None = None,
-             slab_fused: bool | None = None):
-        """Select kernel schedules.  x-pass variants 0, 1 and 3 (FFT) and the
-        slab shape give identical results; the compile-time stage plans (grid
-        edges 16..200) and the fused slab kernels on them agree to fp32
-        rounding (bit-identical at 175); the
+             slab_fused: bool | None = None, pencil_chunks: int | None = None):
+        """Select kernel schedules.  x-pass variants 0, 1 and 3 (FFT), the
+        slab shape and the inverse z pass's chunks per workgroup
+        (pencil_chunks, 0 = automatic) give identical results; the
+        compile-time stage plans (grid edges 16..200) and the fused slab
+        kernels on them agree to fp32 rounding (bit-identical at 175); the
         pencil passes, the radix orders and variant 2 (split-bf16 MFMA DFT)
         agree within 1e-4 relative."""
         if radix_order is not None:
@@ -223,6 +224,8 @@ class SearchPlan:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_STATIC_PLANS, int(bool(static_plans)))
         if slab_fused is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_SLAB_FUSED, int(bool(slab_fused)))
+        if pencil_chunks is not None:
+            nat.call("vx_plan_tune", self._h, nat.VX_TUNE_PENCIL_CHUNKS, int(pencil_chunks))
         if pencil is not None:
             nat.call("vx_plan_tune", self._h, nat.VX_TUNE_FORCE_PENCIL, int(bool(pencil)))
         if xpass_variant is not None:
