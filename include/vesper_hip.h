@@ -134,10 +134,9 @@ int vx_plan_destroy(vx_plan* plan);
                                  through LDS.  Results equal to fp32 rounding (bit-identical
                                  at 175). */
 #define VX_TUNE_PENCIL_CHUNKS 11 /* grid edges 201..400 on the fused pencil passes: chunks of
-                                    line pairs each workgroup of the inverse z pass walks (the
-                                    next chunk's rows load while the current one transforms);
-                                    0 (default) = automatic.  Bit-identical results for every
-                                    value. */
+                                    line pairs each workgroup of the inverse z pass walks (one
+                                    argmax atomic per workgroup); 0 (default) = automatic.
+                                    Bit-identical results for every value. */
 int vx_plan_tune(vx_plan* plan, int key, int value);
 
 /* info_host[8] = {n, mode, batch, groups, query channels, workspace bytes,

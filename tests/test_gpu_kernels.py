@@ -661,9 +661,9 @@ def test_fused_slabs_identical(gpu, n, mode):
 @pytest.mark.parametrize("n,mode", [(208, nat.VX_MODE_VEC), (308, nat.VX_MODE_PROB), (343, nat.VX_MODE_CC)])
 def test_pencil_chunks_identical(gpu, n, mode):
     """The fused inverse z pass walks several chunks of line pairs per
-    workgroup (the next chunk's rows load while the current one
-    transforms); every chunk count gives bit-identical scores and
-    translations, also where prob mixing reads the real maps it writes."""
+    workgroup and publishes one argmax atomic; every chunk count gives
+    bit-identical scores and translations, also where prob mixing reads the
+    real maps it writes."""
     rng = np.random.default_rng(9300 + n)
     c = n / 2
     g = np.indices((n, n, n), dtype=np.float32).transpose(1, 2, 3, 0) - c

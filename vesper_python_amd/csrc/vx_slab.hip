@@ -449,14 +449,13 @@ k_axis_y_fused(float2* __restrict__ buf, FftDesc fdg, int inverse, int ylo, int 
 
 // Inverse c2r along z over chunks of P line pairs of one x-slab (the k_inv_z
 // of vx_search.hip): the Hermitian pack of rows (2l, 2l + 1) of the
-// y-inverted half spectrum happens in the first stage, the later stages run
-// in place with consecutive threads on consecutive lines, and the last stage
-// reduces its outputs into the argmax (and writes the real maps when asked)
-// from registers.  Each workgroup walks K chunks [blockIdx.x K, ...): the raw
-// rows of the next chunk load into registers while the current chunk's
-// stages run, the twiddles are staged once, and one atomic publishes the
-// workgroup's argmax (the same maximum as one workgroup per chunk: the key
-// order is total).
+// y-inverted half spectrum happens inside the first stage's loads from HBM,
+// the later stages run in place with consecutive threads on consecutive
+// lines, and the last stage reduces its outputs into the argmax (and writes
+// the real maps when asked) from registers.  Each workgroup walks K chunks
+// [blockIdx.x K, ...) and publishes one atomic: the keys of a launch share a
+// cache line, where one atomic per chunk serialises (the same maximum either
+// way: the key order is total).
 template <class SP>
 __global__ void __launch_bounds__(256)
 k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, unsigned long long* __restrict__ keys,
@@ -470,9 +469,6 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, 
     constexpr int NS = SP::kNS;
     constexpr int NT = 256;
     constexpr int R0 = SP::kR[0], T0 = n / R0;
-    // first-stage butterflies per thread: a chunk holds at most
-    // kPencilMaxV NT / n line pairs (the launcher's bound on P)
-    constexpr int RND = (T0 * (kPencilMaxV * NT / n) + NT - 1) / NT;
     extern __shared__ float2 lds[];
     __shared__ float s_v[16];
     __shared__ uint32_t s_i[16];
@@ -483,46 +479,35 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, 
     const int c0 = blockIdx.x * K;
     const int c1 = min(c0 + K, (npair + P - 1) / P);
     const float2* src = sum + ((size_t)bg * n + x) * (size_t)n * nhz;
-    // rows 2l and 2l + 1 at each first-stage input's half-spectrum index
-    float2 ra[RND][R0], rb[RND][R0];
-    auto load = [&](int c) {
-        const int l0 = c * P, nb = T0 * min(P, npair - l0);
-#pragma unroll
-        for (int r = 0; r < RND; ++r) {
-            const int b = r * NT + (int)threadIdx.x;
-            if (b < nb) {
-                const int ll = b / T0, i = b - ll * T0, l = l0 + ll;
-                const bool hasb = (2 * l + 1) < n;
-#pragma unroll
-                for (int j = 0; j < R0; ++j) {
-                    const int k = i + j * T0;
-                    const int kk = (k <= half) ? k : n - k;
-                    ra[r][j] = src[(size_t)(2 * l) * nhz + kk];
-                    rb[r][j] = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
-                }
-            }
-        }
-    };
     float bv = -INFINITY;
     uint32_t bi = 0xffffffffu;
     const size_t n3 = (size_t)n * n * n;
     const bool maps = real_out != nullptr;
-    load(c0);
 #pragma unroll 1
     for (int c = c0; c < c1; ++c) {
         const int l0 = c * P, np = min(P, npair - l0);
         {
             const int nb = T0 * np;
-#pragma unroll
-            for (int r = 0; r < RND; ++r) {
-                const int b = r * NT + (int)threadIdx.x;
+#pragma unroll 1
+            for (int b0 = 0; b0 < nb; b0 += NT) {
+                const int b = b0 + (int)threadIdx.x;
                 if (b < nb) {
-                    const int ll = b / T0, i = b - ll * T0;
+                    const int ll = b / T0, i = b - ll * T0, l = l0 + ll;
+                    const bool hasb = (2 * l + 1) < n;
+                    // every load of the butterfly first, then the pack
+                    float2 ra[R0], rb[R0];
+#pragma unroll
+                    for (int j = 0; j < R0; ++j) {
+                        const int k = i + j * T0;
+                        const int kk = (k <= half) ? k : n - k;
+                        ra[j] = src[(size_t)(2 * l) * nhz + kk];
+                        rb[j] = hasb ? src[(size_t)(2 * l + 1) * nhz + kk] : make_float2(0.f, 0.f);
+                    }
                     float2 v[R0];
 #pragma unroll
                     for (int j = 0; j < R0; ++j) {
                         const int k = i + j * T0;
-                        float2 a = ra[r][j], bb = rb[r][j];
+                        float2 a = ra[j], bb = rb[j];
                         if (k <= half) {
                             if (k == 0 || (even && k == half)) { a.y = 0.f; bb.y = 0.f; }
                             v[j] = make_float2(a.x - bb.y, -a.y - bb.x);       // conj(A + iB)
@@ -537,7 +522,6 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, 
                 }
             }
         }
-        if (c + 1 < c1) load(c + 1);
         __syncthreads();
         slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS - 1>(A, fd.twid, np, 1.0f / (float)np);
         {
