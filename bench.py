@@ -134,8 +134,8 @@ def cpu_baseline(n1, n2, angles, budget_s: float, nrows_max: int):
 
 
 def cpu_stages(w, n1, n2, planes: int = 8):
-    """The steps beside the search loop on one host core, as SURVEY's CPU
-    column times them: fastVEC of the query map by the oracle's C
+    """The steps beside the search loop on one host core (the reference runs
+    them on the CPU once per search): fastVEC of the query map by the oracle's C
     restatement of doFastVEC (exact order, float64) on `planes` output
     x-planes spread over the grid, extrapolated to all of them; and the FP64
     get_score at one translation by the oracle (NumPy).  The same calls on
