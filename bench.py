@@ -798,19 +798,20 @@ def measure_ablation(n_train=16, n_test=8, s=3.0, g=16.0, label_ang=10.0, judge_
                      fracs=(0.0025, 0.005, 0.01, 0.05), seed_train=6000, seed_test=7000, epochs=8,
                      workers=16, weights_in=None, weights_out=None):
     """What the learned proposer gains over the same search without it (the
-    config-5 protocol; its full size is 512 training and 64 held-out pairs):
-    the network is pretrained on the exhaustive -A label_ang landscapes of
-    n_train pairs from the uneven generator (streamed: one pair's plan on the
-    device at a time), then on each of n_test held-out pairs three policies
-    spend the same budget -- the -A coarse_ang landscape, a fraction (each
-    of `fracs`) of the -A judge_ang rows in two shares, three neighbour
-    sweeps around the best rows -- choosing the shares by (mlp) the network
-    refined on the pair's coarse landscape, (coarse) the coarse landscape's
-    score of each row's nearest coarse row, (random) uniform draws.  Judge:
-    the exhaustive -A judge_ang search of the pair (top-10 recall, and the
-    gap between the best score found and the exhaustive best).  Maps
-    resampled at -s s (2x the config's 1.5 A, so that the protocol fits one
-    GPU call).  weights_out saves the pretrained network (.npz) and
+    learned-proposal protocol; its full size is 512 training and 64
+    held-out pairs): the network is pretrained on the exhaustive -A
+    label_ang landscapes of n_train pairs from the uneven generator
+    (streamed: one pair's plan on the device at a time), then on each of
+    n_test held-out pairs four policies spend the same budget -- the -A
+    coarse_ang landscape, a fraction (each of `fracs`) of the -A judge_ang
+    rows in two shares, three neighbour sweeps around the best rows --
+    choosing the shares by (mlp) the network refined on the pair's coarse
+    landscape, (coarse) the coarse landscape's score of each row's nearest
+    coarse row, (random) uniform draws, (hybrid) half of each share by the
+    coarse ranking and the rest by the network.  Judge: the exhaustive -A
+    judge_ang search of the pair (top-10 recall, and the gap between the
+    best score found and the exhaustive best).  Maps resampled at -s s (2x
+    the uneven workload's 1.5 A, so that the protocol fits one GPU call).  weights_out saves the pretrained network (.npz) and
     weights_in loads one instead of pretraining, so that training and the
     held-out evaluation can run separately."""
     import torch

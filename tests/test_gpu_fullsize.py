@@ -10,7 +10,7 @@ arithmetic, ref32) re-scores
   and no sampled row beats the GPU's 10th score -- a row the GPU under-scored
   out of its top ten would show here;
 
-and the best row against FP64 (ref64).  Prob mode runs the config-3 inputs:
+and the best row against FP64 (ref64).  Prob mode runs four probability inputs per map:
 Dirichlet(0.5) channels on the density mask, Gaussian-smoothed and
 renormalised, read from .npy and smoothed by fastVEC as the prob command
 does (main.py:302-361)."""
@@ -121,13 +121,13 @@ def test_pair200_top10_and_sample_against_oracle(gpu, pair200):
 
 @pytest.mark.parametrize("mode", ["Overlap", "CC", "PCC"])
 def test_pair200_scalar_modes_against_oracle(gpu, pair200, mode):
-    """Config 2's modes O, C and P on the full-size pair."""
+    """Modes O, C and P on the full-size pair."""
     _, _, n1, n2 = pair200
     _check_search(n1, n2, mode)
 
 
-def test_pair200_prob_mode_config3_generator(gpu, pair200, fastvec_ref):
-    """Config 3: four Dirichlet(0.5) channels per map (smoothed, renormalised,
+def test_pair200_prob_mode_dirichlet_generator(gpu, pair200, fastvec_ref):
+    """Prob mode: four Dirichlet(0.5) channels per map (smoothed, renormalised,
     zero off the density mask), stored as the (nz, ny, nx, 4) .npy the prob
     command reads, resampled and smoothed by fastVEC (checked against the C
     oracle), then the two-pass mixed search; the top 10 by mixed score and
@@ -138,7 +138,7 @@ def test_pair200_prob_mode_config3_generator(gpu, pair200, fastvec_ref):
     synth.add_prob_channels(rng, pair.b)
     tmp = tempfile.mkdtemp(prefix="vx_prob_")
     pp = synth.write_pair(pair, os.path.join(tmp, "p"))
-    print("[case] config3 resample + fastVEC of 10 maps", flush=True)
+    print("[case] prob resample + fastVEC of 10 maps", flush=True)
     with contextlib.redirect_stdout(io.StringIO()):
         n1, n2, n1p, n2p = cli.resample_prob(pp["A"], pp["B"], pp["A_prob"], pp["B_prob"], pair.a.threshold,
                                              pair.b.threshold, 2.0, 16.0)
@@ -161,14 +161,14 @@ def test_pair200_prob_mode_config3_generator(gpu, pair200, fastvec_ref):
     plan = engine.SearchPlan(n, nat.VX_MODE_PROB, batch=16)
     plan.set_target(*n1._dev, engine.to_dev(tp, torch.float32), 0.0)
     plan.set_search(*n2._dev, engine.to_dev(sp, torch.float32), 0.0)
-    print("[case] config3 pass 1", flush=True)
+    print("[case] prob pass 1", flush=True)
     plan.set_prob_stats(0.5, 1.0, 0.0, 1.0, 0.0, enable=False)
     b1, _ = plan.score_rows(rot)
     b1 = b1.double().cpu().numpy() / n ** 3
     vave, vstd = float(b1[:, 0].mean()), float(b1[:, 0].std())
     pave, pstd = float(b1[:, 1].mean()), float(b1[:, 1].std())
     alpha = 0.5
-    print("[case] config3 pass 2 (mixed)", flush=True)
+    print("[case] prob pass 2 (mixed)", flush=True)
     # the reference mixes the unscaled correlation maps with the rd3-scaled
     # statistics (VESPER_1_prob_new.py:920-926, :1530-1545)
     plan.set_prob_stats(alpha, vstd, vave, pstd, pave, enable=True)
@@ -183,7 +183,7 @@ def test_pair200_prob_mode_config3_generator(gpu, pair200, fastvec_ref):
     top = [int(x) for x in rows]
     want = list(dict.fromkeys(top + sample))
     with OraclePool(sdata, n2.vec, spec, prob=[sp[k] for k in range(4)]) as pool:
-        oracle = pool.prob_rows(want, angles, alpha, vstd, vave, pstd, pave, label="config3")
+        oracle = pool.prob_rows(want, angles, alpha, vstd, vave, pstd, pave, label="prob")
     s10 = min(oracle[r][0] for r in top)
     for r in want:
         ms, mt, vs, vt, ps = oracle[r]

@@ -1,5 +1,5 @@
 """The drop-in CLI's printed output against the reference's print
-statements restated over the oracle's results (config 1: the 100^3/60^3
+statements restated over the oracle's results (the 100^3/60^3
 pair, -g 16 -s 7 -A 30 -N 10).
 
 Every line is compared as text with its numbers cut out, and number by

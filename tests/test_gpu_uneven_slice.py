@@ -1,4 +1,4 @@
-"""A slice of the uneven multi-pair workload (config 4): four pairs from the
+"""A slice of the uneven multi-pair workload: four pairs from the
 uneven generator (A edge U{96..320} at 1 A, 40-60 % sub-chains) resampled at
 -s 1.5 -g 16, grid edges 140, 175, 220 and 260 (slab and pencil FFT paths),
 searched exhaustively at -A 10 by one batched call (one plan and captured
@@ -32,7 +32,7 @@ def slice4(tmp_path_factory):
     d = tmp_path_factory.mktemp("cfg4")
     pairs = []
     for seed, edge in SEEDS.items():
-        print("[case] config4 pair", seed, flush=True)
+        print("[case] uneven slice pair", seed, flush=True)
         p = synth.make_uneven_pair(seed)
         paths = synth.write_pair(p, str(d / f"p{seed}"))
         with contextlib.redirect_stdout(io.StringIO()):
@@ -41,13 +41,13 @@ def slice4(tmp_path_factory):
             n2 = V.fastVEC(m2, n2, 16.0)
         assert int(n1.xdim) == edge, (seed, n1.xdim)
         pairs.append((n1, n2))
-    print("[case] config4 batched search", flush=True)
+    print("[case] uneven slice batched search", flush=True)
     res = batch.search_pairs(pairs, ang=10.0, topn=10, use_graph=True)
     return pairs, res
 
 
 @pytest.mark.parametrize("k", range(len(SEEDS)))
-def test_config4_pair_against_oracle(gpu, slice4, k):
+def test_uneven_slice_pair_against_oracle(gpu, slice4, k):
     pairs, res = slice4
     n1, n2 = pairs[k]
     r = res[k]
@@ -61,7 +61,7 @@ def test_config4_pair_against_oracle(gpu, slice4, k):
     top = [int(x) for x in r["top"]]
     sample = list(range(0, len(angles), 6151))
     with OraclePool(sdata, n2.vec, spec) as pool:
-        oracle = pool.v_rows(list(dict.fromkeys(top + sample)), angles, "VecProduct", label=f"config4 N={n}")
+        oracle = pool.v_rows(list(dict.fromkeys(top + sample)), angles, "VecProduct", label=f"uneven slice N={n}")
     s10 = min(oracle[row][0] for row in top)
     for row in top + sample:
         ob, ot, _ = oracle[row]
@@ -75,7 +75,7 @@ def test_config4_pair_against_oracle(gpu, slice4, k):
             assert ob <= s10 + RTOL * abs(s10), (n, row, ob, s10)
 
 
-def test_config4_split_bf16_mfma_against_fp32(gpu, slice4):
+def test_uneven_slice_split_bf16_mfma_against_fp32(gpu, slice4):
     pairs, res = slice4
     angles = calc_angle_comb(10.0)[::7]
     rot = torch.from_numpy(rotation_rows(angles)).cuda()
