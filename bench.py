@@ -936,6 +936,8 @@ def main():
     ap.add_argument("--ablation-fracs", default="0.0025,0.005,0.01,0.05",
                     help="shares of the -A 5 rows each policy may FFT-score")
     ap.add_argument("--ablation-epochs", type=int, default=8)
+    ap.add_argument("--ablation-test-seed", type=int, default=7000,
+                    help="seed of the first held-out pair (the training pairs start at 6000)")
     ap.add_argument("--ablation-weights-out", default="", help="save the pretrained network (.npz)")
     ap.add_argument("--ablation-weights-in", default="", help="load a pretrained network instead of training")
     ap.add_argument("--aux", action="store_true",
@@ -958,7 +960,7 @@ def main():
         ntr, nte = (int(v) for v in args.ablation_pairs.split(","))
         print(json.dumps({"ablation": measure_ablation(
             n_train=ntr, n_test=nte, fracs=tuple(float(v) for v in args.ablation_fracs.split(",")),
-            epochs=args.ablation_epochs, weights_in=args.ablation_weights_in or None,
+            epochs=args.ablation_epochs, seed_test=args.ablation_test_seed, weights_in=args.ablation_weights_in or None,
             weights_out=args.ablation_weights_out or None)}), flush=True)
         return
     if args.pairs48:
