@@ -313,12 +313,21 @@ k_inv_yz_fused(const float2* __restrict__ sum, FftDesc fdg, int G, unsigned long
 // with the same folding of boundary passes into the stages next to them.
 // ---------------------------------------------------------------------------
 
+// The line pairs of a pencil chunk (one per line of the z transform): the
+// chunk's lines sit interleaved in LDS, element e of line l at e kPL + l
+// (the tile layout), so that the lanes of a stage reach consecutive
+// addresses whatever the line count (line-major lines of odd length N at
+// 5..10 lines per chunk hit the same LDS banks).
+template <class SP>
+constexpr int pencil_lines() { return kPencilValues / SP::kN; }
+
 // Forward r2c along z for a chunk of P line pairs of one x-slab (the
 // k_fwd_z of vx_search.hip): the first stage reads its butterflies' two real
 // rows from the planes (consecutive threads on consecutive butterflies of a
 // line: coalesced, zero outside the box's z range without a load), the later
-// stages run in place with consecutive threads on consecutive lines, then
-// the unpack writes the rows' half spectra.
+// stages run in place on the interleaved lines, then the unpack writes the
+// rows' half spectra.  A last chunk of fewer than P pairs transforms
+// leftover lines too and never reads them.
 template <class SP>
 __global__ void __launch_bounds__(256)
 k_fwd_z_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fdg, int lo, int w, int P) {
@@ -327,6 +336,7 @@ k_fwd_z_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd
     constexpr int nhz = n / 2 + 1;
     constexpr int NS = SP::kNS;
     constexpr int NT = 256;
+    constexpr int PL = pencil_lines<SP>();   // == P (the launcher's check)
     extern __shared__ float2 lds[];
     float2* A = lds;
     const FftDesc fd = stage_twiddles(fdg, lds + P * n);
@@ -359,19 +369,19 @@ k_fwd_z_fused(const float* __restrict__ q, float2* __restrict__ spec, FftDesc fd
                     v[j] = make_float2(re, im);
                 }
                 dft<R>(v);
-                float2* O = A + ll * n + i * R;   // nothing reads A before the barrier below
+                float2* O = A + i * R * PL + ll;   // nothing reads A before the barrier below
 #pragma unroll
-                for (int j = 0; j < R; ++j) O[j] = v[j];
+                for (int j = 0; j < R; ++j) O[j * PL] = v[j];
             }
         }
         __syncthreads();
     }
-    slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS>(A, fd.twid, np, 1.0f / (float)np);
+    slab_stages<SP, NT, kPencilMaxV, kLayTile, PL, 1, NS>(A, fd.twid, 0, 0.f);
     float2* out = spec + (plane * n + x) * (size_t)n * nhz;
     for (int u = threadIdx.x; u < np * nhz; u += NT) {
         const int ll = u / nhz, k = u - ll * nhz, l = l0 + ll;
-        const float2 zk = A[ll * n + k];
-        const float2 wv = A[ll * n + (k == 0 ? 0 : n - k)];
+        const float2 zk = A[k * PL + ll];
+        const float2 wv = A[(k == 0 ? 0 : n - k) * PL + ll];
         out[(size_t)(2 * l) * nhz + k] = make_float2(0.5f * (zk.x + wv.x), 0.5f * (zk.y - wv.y));
         if (2 * l + 1 < n) out[(size_t)(2 * l + 1) * nhz + k] = make_float2(0.5f * (zk.y + wv.y), 0.5f * (wv.x - zk.x));
     }
@@ -450,9 +460,9 @@ k_axis_y_fused(float2* __restrict__ buf, FftDesc fdg, int inverse, int ylo, int 
 // Inverse c2r along z over chunks of P line pairs of one x-slab (the k_inv_z
 // of vx_search.hip): the Hermitian pack of rows (2l, 2l + 1) of the
 // y-inverted half spectrum happens inside the first stage's loads from HBM,
-// the later stages run in place with consecutive threads on consecutive
-// lines, and the last stage reduces its outputs into the argmax (and writes
-// the real maps when asked) from registers.  Each workgroup walks K chunks
+// the later stages run in place on the interleaved lines (pencil_lines),
+// and the last stage reduces its outputs into the argmax (and writes the
+// real maps when asked) from registers.  Each workgroup walks K chunks
 // [blockIdx.x K, ...) and publishes one atomic: the keys of a launch share a
 // cache line, where one atomic per chunk serialises (the same maximum either
 // way: the key order is total).
@@ -469,6 +479,7 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, 
     constexpr int NS = SP::kNS;
     constexpr int NT = 256;
     constexpr int R0 = SP::kR[0], T0 = n / R0;
+    constexpr int PL = pencil_lines<SP>();   // == P (the launcher's check)
     extern __shared__ float2 lds[];
     __shared__ float s_v[16];
     __shared__ uint32_t s_i[16];
@@ -516,30 +527,31 @@ k_inv_z_fused(const float2* __restrict__ sum, FftDesc fdg, int G, int P, int K, 
                         }
                     }
                     dft<R0>(v);
-                    float2* O = A + ll * n + i * R0;   // the previous chunk's reads of A ended at a barrier
+                    float2* O = A + i * R0 * PL + ll;   // the previous chunk's reads of A ended at a barrier
 #pragma unroll
-                    for (int j = 0; j < R0; ++j) O[j] = v[j];
+                    for (int j = 0; j < R0; ++j) O[j * PL] = v[j];
                 }
             }
         }
         __syncthreads();
-        slab_stages<SP, NT, kPencilMaxV, kLayLinesLF, 0, 1, NS - 1>(A, fd.twid, np, 1.0f / (float)np);
+        slab_stages<SP, NT, kPencilMaxV, kLayTile, PL, 1, NS - 1>(A, fd.twid, 0, 0.f);
         {
             constexpr int S = NS - 1, R = SP::kR[S], P = SP::P(S), T = n / R;
             static_assert(T == P, "the last stage of a plan spans its earlier radices");
-            const int nb = T * np;
-            const float npinv = 1.0f / (float)np;
+            // maps: consecutive threads along a line (coalesced stores);
+            // else along the lines (consecutive LDS addresses)
+            const int nb = maps ? T * np : T * PL;
 #pragma unroll 1
             for (int b0 = 0; b0 < nb; b0 += NT) {
                 const int b = b0 + (int)threadIdx.x;
-                if (b < nb) {
-                    int ll, i;
-                    if (maps) { ll = b / T; i = b - ll * T; }
-                    else      { i = fdiv(b, np, npinv); ll = b - imul(i, np); }
+                int ll, i;
+                if (maps) { ll = b / T; i = b - ll * T; }
+                else      { i = b / PL; ll = b - i * PL; }
+                if (b < nb && ll < np) {
                     const int l = l0 + ll;
                     float2 v[R];
 #pragma unroll
-                    for (int j = 0; j < R; ++j) v[j] = A[ll * n + i + j * T];
+                    for (int j = 0; j < R; ++j) v[j] = A[(i + j * T) * PL + ll];
                     twiddle_dft<R, P, SP::TW(S)>(v, i, fd.twid);
                     const uint32_t row = ((uint32_t)x * n + 2 * l) * n;
                     const bool hasb = 2 * l + 1 < n;
@@ -650,7 +662,8 @@ bool launch_fwd_z_fused(const float* q, float2* spec, const FftDesc& fd, int n, 
                         hipStream_t st, int* rc) {
     *rc = VX_OK;
     const int P = pencil_pairs(n);
-    if (n * P > 256 * kPencilMaxV) return false;
+    // the kernels interleave pencil_lines = kPencilValues / n lines
+    if (n * P > 256 * kPencilMaxV || P != kPencilValues / n) return false;
     const int l0 = lo >> 1, l1 = ((lo + w - 1) >> 1) + 1;
     const dim3 gz((l1 - l0 + P - 1) / P, w, nplanes);
     const size_t lds = ((size_t)P * n + fd.ntw) * sizeof(float2);
@@ -706,7 +719,7 @@ bool launch_inv_z_fused(const float2* sum, const FftDesc& fd, int n, int nbg, in
                         int nkeys, float* real_out, hipStream_t st, int* rc, int chunks) {
     *rc = VX_OK;
     const int P = pencil_pairs(n);
-    if (n * P > 256 * kPencilMaxV) return false;
+    if (n * P > 256 * kPencilMaxV || P != kPencilValues / n) return false;
     const int npair = (n + 1) / 2;
     const int nch = (npair + P - 1) / P;
     const int K = pencil_chunks(nch, n, nbg, chunks);
