@@ -3,8 +3,10 @@
 // stage from the query planes (k_fwd_zy_fused), the inverse slab reducing its
 // last stage into the argmax (k_inv_yz_fused), and the same three folds in the
 // pencil passes of the larger grids (k_fwd_z_fused, k_axis_y_fused,
-// k_inv_z_fused).  The separate-pass forms and the runtime-plan kernels live
-// in vx_search.hip; these are the default on every grid edge with a
+// k_inv_z_fused; the z passes keep a chunk's lines interleaved in LDS, and
+// the inverse one walks several chunks per workgroup with one argmax atomic).
+// The separate-pass forms and the runtime-plan kernels live in
+// vx_search.hip; these are the default on every grid edge with a
 // compile-time plan (VX_TUNE_SLAB_FUSED).
 // Their own translation unit, so that they compile beside the search
 // kernels.
