@@ -1,5 +1,6 @@
 """Host-side logic of the package on the CPU: the Euler-grid indexing the
-proposer's neighbour sweep relies on (brute force over the grid), the bench's
+proposer's neighbour sweep and coarse-row ranking rely on (brute force over
+the grid), the bench's
 one-GPU contract and byte model, the traffic line it reads from the committed
 profile, the batch and autotuner choices, and the compile-time FFT plan list
 against the descriptor's stage order."""
@@ -27,6 +28,36 @@ def test_grid_shape_rejects_other_row_sets():
     a = calc_angle_comb(30.0)
     assert proposer.grid_shape(a[::-1]) is None
     assert proposer.grid_shape(a[:-1]) is None
+
+
+def test_nearest_coarse_rows_brute_force():
+    """The coarse and hybrid proposal policies rank a fine row by its nearest
+    row of the coarse grid: each returned row is a calc_angle_comb row of the
+    coarse step, each of its angles is at the smallest distance any coarse
+    value of that axis reaches (x and y wrap at 360, z does not; midpoints
+    may go either way), and the distance returned is their Euclidean norm."""
+    fine, coarse = calc_angle_comb(5.0), calc_angle_comb(30.0)
+    rng = np.random.default_rng(3)
+    pick = fine[rng.choice(len(fine), 400, replace=False)]
+    rows, dist = proposer.nearest_coarse_rows(pick, 30.0)
+    got = coarse[rows]
+    wrap = lambda d: np.abs((d + 180.0) % 360.0 - 180.0)
+    axes = (np.arange(0, 360, 30.0), np.arange(0, 360, 30.0), np.arange(0, 180.0 + 1e-9, 30.0))
+    dev = np.empty_like(pick)
+    for ax, vals in enumerate(axes):
+        d_all = pick[:, ax, None] - vals[None, :]
+        d_all = wrap(d_all) if ax < 2 else np.abs(d_all)
+        d_got = pick[:, ax] - got[:, ax]
+        d_got = wrap(d_got) if ax < 2 else np.abs(d_got)
+        assert np.allclose(d_got, d_all.min(axis=1)), ax
+        dev[:, ax] = d_got
+    assert np.allclose(dist, np.linalg.norm(dev, axis=1))
+
+
+def test_propose_rounds_rejects_unknown_policy():
+    assert "hybrid" in proposer.POLICIES
+    with pytest.raises(ValueError):
+        proposer.propose_rounds(None, None, None, policy="best")
 
 
 # ------------------------------------------------------------ bench and plans
