@@ -1,9 +1,8 @@
 """Host-side logic of the package on the CPU: the Euler-grid indexing the
 proposer's neighbour sweep and coarse-row ranking rely on (brute force over
-the grid), the bench's
-one-GPU contract and byte model, the traffic line it reads from the committed
-profile, the batch and autotuner choices, and the compile-time FFT plan list
-against the descriptor's stage order."""
+the grid), the bench's one-GPU contract and byte model, the traffic line it
+reads from the committed profile, the batch and autotuner choices, and the
+compile-time FFT plan list against the descriptor's stage order."""
 import numpy as np
 import pytest
 
